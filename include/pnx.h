@@ -1,0 +1,137 @@
+/* pnx -- MI355X-native PN-CSD forward solve and adjoint (C ABI).
+ *
+ * Drop-in for the reference's k-ratio hot path.  A Julia host binds these with
+ * ccall; the Python package pnepma_amd binds them with
+ * ctypes.  Plain C types only; every call returns 0 on success and a negative
+ * status on failure, with a thread-local message in pnx_last_error().
+ *
+ * Layouts at this boundary are the reference's: per-class arrays of points
+ * (x fastest, then y, then z; the class point counts are the reference's
+ * staggered sizes) with the per-point vector (elements, moments or lines)
+ * fastest.  The library converts to its device layout internally.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   pnx_problem_create      ForwardPNProblem{N,T}(...)          pn_solver/pn_problem.jl:72-106
+ *   pnx_forward             calc_intensities(problem, m, ...)   pn_solver/pn_intensities.jl:325-351
+ *   pnx_forward_adjoint     Zygote.@adjoint calc_intensities    pn_solver/pn_intensities.jl:353-413
+ *   pnx_step                step_pde!(problem, u, du, t, dt)    starmap/src/solver.jl:171-196
+ */
+#ifndef PNX_H
+#define PNX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PNX_OK = 0,
+  PNX_ERR_ARG = -1,      /* invalid argument */
+  PNX_ERR_DEVICE = -2,   /* HIP runtime error */
+  PNX_ERR_MEMORY = -3,   /* device allocation failed or budget exceeded */
+  PNX_ERR_STATE = -4,    /* call out of order (e.g. solve before inputs are set) */
+  PNX_ERR_UNSUPPORTED = -5
+};
+
+typedef struct pnx_context pnx_context;
+typedef struct pnx_problem pnx_problem;
+
+/* Geometry and sizes of one solve batch: n_beams beams that share grid,
+ * material and energy grid (ForwardPNProblem{N,Float64}). */
+typedef struct {
+  int32_t N;          /* PN order, 1..9 (odd orders are compiled) */
+  int32_t M[3];       /* grid points per axis; 1 = singleton; non-singleton axes first */
+  double lo[3];       /* axis extents in cm (singleton: position) */
+  double hi[3];
+  int32_t n_elements; /* 1..8 */
+  int32_t n_lines;    /* 1..8 */
+  int32_t n_steps;    /* energy steps */
+  int32_t n_beams;    /* beams in the batch */
+  double E_initial;   /* eV */
+  double E_cutoff;    /* eV */
+  int32_t ckpt_every; /* adjoint checkpoint interval in energy steps (0 = n_steps) */
+} pnx_desc;
+
+const char* pnx_last_error(void);
+int pnx_version(void);
+
+/* Context: one device, one HIP stream. */
+int pnx_context_create(int device, pnx_context** out);
+int pnx_context_destroy(pnx_context* ctx);
+int pnx_context_synchronize(pnx_context* ctx);
+
+/* Problem: allocates the device state of the batch. */
+int pnx_problem_create(pnx_context* ctx, const pnx_desc* desc, pnx_problem** out);
+int pnx_problem_destroy(pnx_problem* p);
+/* Device bytes held by the problem (state, checkpoints, inputs). */
+int64_t pnx_problem_device_bytes(const pnx_problem* p);
+
+/* Energy-point records (n_steps + 1 each, record length
+ * n_e + n_e*(N+1) + 1 + n_lines doubles: s[e], c[e][l], gfac, sigma[r]).
+ *   fwd[k]  : coefficients at E_{k-1}-dE/2, sigma at E_k     (pn_intensities.jl:341-349)
+ *   adj[i]  : adjoint coefficients and sigma at E_i + dE/2   (pn_intensities.jl:397-399)
+ *   dens[i] : forward coefficients, gfac, sigma at E_i       (pn_intensities.jl:394-395)
+ * adj and dens may be NULL for forward-only use. */
+int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj, const double* dens);
+
+/* Material inputs.  rho[c] (c = 0..7, eee..ooo) is [point][element] over the
+ * reference's class-c points; entries for classes the grid never updates may be
+ * NULL.  W is [eee point][line] = absorption factor * number of atoms of the
+ * line's element (x_ray_intensities.jl:183). */
+int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const double* W);
+
+/* Beam source.  mms[c] = beam moments of each x-odd class c (n_c values),
+ * ppos[b*8 + c] = position pdf over the (y, z) points of class c for beam b
+ * (y fastest); NULL for x-even classes (pn_problem.jl:192-203). */
+int pnx_problem_set_beams(pnx_problem* p, const double* const mms[8], const double* const* ppos);
+
+/* calc_intensities for every beam: I_out[b*n_lines + r]. */
+int pnx_forward(pnx_problem* p, double* I_out);
+
+/* Forward pass, then the reference adjoint sweep with checkpointing every
+ * ckpt_every steps and forward recomputation between checkpoints.
+ *   I_bar[b*n_lines + r]   seed of the pullback per beam
+ *   I_out[b*n_lines + r]   intensities
+ *   rho_bar[c]             [point][element] per class, summed over beams (NULL to skip a class)
+ *   gen_out                [eee point][line], summed over beams:
+ *                          sum_i u_i,eee(p,0) * dE*dvol*sigma_r(E_i)*int_fac_i*I_bar_r,
+ *                          from which ac_bar = noa[el_r] * gen and noa_bar[el_r] += ac * gen
+ *                          (x_ray_intensities.jl:193-223). May be NULL. */
+int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out);
+
+/* One energy step (step_pde!) on a host state for beam `beam`: u[c] is the
+ * reference's [point][moment] array of class c (ghost cells ignored on input,
+ * undefined on output); classes the grid never updates may be NULL.
+ * adjoint != 0 runs the adjoint problem with record index `record` of adj and
+ * the eee m=0 source -sum_r sigma_r W_r I_bar_r (I_bar may be NULL for none);
+ * otherwise the forward problem with record `record` of fwd. */
+int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_bar, double* const u[8]);
+
+/* Timing support: run the forward march `reps` times on device-resident
+ * inputs without host copies; returns the elapsed milliseconds of the last
+ * run measured with HIP events on the problem's stream. */
+int pnx_forward_timed(pnx_problem* p, int reps, int use_graph, double* ms_out);
+int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out);
+
+/* Average duration (ms) of one launch of the fused half-step kernel of the
+ * given parity (0 even, 1 odd), measured with HIP events around `reps`
+ * back-to-back launches on the problem's stream, and the algorithmic bytes
+ * of one such launch. */
+int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, double* bytes_out);
+
+/* Matrices the kernels use, for inspection: the coefficient buffer of order N
+ * (adjoint = 0 forward, 1 adjoint): all advection nonzeros, then dense
+ * boundary blocks.  Returns the number of doubles when out is NULL. */
+int pnx_coefficients(int N, int adjoint, double* out, int capacity);
+/* Dense advection block A_d[c] (n_c x n_switch, forward sign) and boundary
+ * block B_d[c] (n_c x n_switch, c odd in d) as the library builds them. */
+int pnx_advection_block(int N, int c, int d, double* out);
+int pnx_boundary_block(int N, int c, int d, int adjoint, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

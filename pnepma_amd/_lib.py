@@ -1,0 +1,77 @@
+"""ctypes binding of libpnx.so (the C ABI declared in include/pnx.h).
+
+The library is built in-tree (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C pnepma_amd/csrc``).  Loading fails loudly if it is missing: there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpnx.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class Desc(C.Structure):
+    _fields_ = [("N", C.c_int32), ("M", C.c_int32 * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("n_elements", C.c_int32), ("n_lines", C.c_int32), ("n_steps", C.c_int32), ("n_beams", C.c_int32),
+                ("E_initial", C.c_double), ("E_cutoff", C.c_double), ("ckpt_every", C.c_int32)]
+
+
+# name -> (restype, argtypes); every symbol include/pnx.h declares
+SIGNATURES = {
+    "pnx_last_error": (C.c_char_p, []),
+    "pnx_version": (C.c_int, []),
+    "pnx_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pnx_context_destroy": (C.c_int, [C.c_void_p]),
+    "pnx_context_synchronize": (C.c_int, [C.c_void_p]),
+    "pnx_problem_create": (C.c_int, [C.c_void_p, C.POINTER(Desc), C.POINTER(C.c_void_p)]),
+    "pnx_problem_destroy": (C.c_int, [C.c_void_p]),
+    "pnx_problem_device_bytes": (C.c_int64, [C.c_void_p]),
+    "pnx_problem_set_records": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "pnx_problem_set_material": (C.c_int, [C.c_void_p, _dp * 8, _dp]),
+    "pnx_problem_set_beams": (C.c_int, [C.c_void_p, _dp * 8, C.POINTER(_dp)]),
+    "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
+    "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, _dp * 8, _dp]),
+    "pnx_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp * 8]),
+    "pnx_forward_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "pnx_forward_adjoint_timed": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "pnx_half_step_bench": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pnx_coefficients": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int]),
+    "pnx_advection_block": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),
+    "pnx_boundary_block": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+}
+
+_lib = None
+
+
+class PnxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpnx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PnxError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C pnepma_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise PnxError(f"pnx error {rc}: {lib().pnx_last_error().decode()}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None

@@ -1,0 +1,753 @@
+// C ABI of the PN-CSD solver: contexts, problems, and the energy-march drivers.
+//
+// The drivers follow the reference loops: calc_intensities
+// (pn_solver/pn_intensities.jl:325-351) and its Zygote adjoint
+// (pn_solver/pn_intensities.jl:353-413), with the full state tape of the
+// reference replaced by checkpoints every `ckpt_every` energy steps and a
+// forward recomputation of each segment during the reverse sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pnx.h"
+#include "pnx_kernels.hpp"
+#include "pnx_tables.hpp"
+
+namespace pnx {
+std::vector<double> build_coefficients(int N, bool adjoint);
+double advection_coefficient(int l, int k, int lf, int kf, int d);
+}  // namespace pnx
+
+using namespace pnx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                               \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess) return fail(PNX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool order_supported(int N) { return N == 1 || N == 3 || N == 5 || N == 7 || N == 9; }
+
+struct DevBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    if (p && n >= count) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return 0;
+    if (hipMalloc(&p, count * sizeof(double)) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(PNX_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(double)) + " bytes failed");
+    }
+    n = count;
+    return 0;
+  }
+};
+
+}  // namespace
+
+struct pnx_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct pnx_problem {
+  pnx_context* ctx = nullptr;
+  pnx_desc d{};
+  Layout L{};
+  Tables T{};
+  int rec_len = 0;
+  int sigma_off = 0;
+  double dE = 0.0;
+  int nck = 0;          // checkpoints
+  int K = 0;            // checkpoint interval
+  long rho_off[8]{};
+  long rho_total = 0;
+  long rb_off[8]{};
+  long rb_stride = 0;   // per beam rho_bar doubles
+  long prof_stride = 0;
+  long partial_step = 0, partial_beam = 0;
+  int class_n[8][3]{};  // reference class sizes
+  bool have_rec = false, have_adj = false, have_mat = false, have_beams = false;
+  DevBuf state, rho, W, coef_f, coef_a, rec_f, rec_a, rec_d, prof, mms, partial, I_dev;
+  DevBuf lam, ckpt, seg, rho_bar_b, gen_b, ibar, sum_buf;
+  hipGraphExec_t fwd_graph = nullptr;
+  std::vector<double> host_tmp;
+};
+
+static void destroy_graph(pnx_problem* p) {
+  if (p->fwd_graph) {
+    (void)hipGraphExecDestroy(p->fwd_graph);
+    p->fwd_graph = nullptr;
+  }
+}
+
+extern "C" {
+
+const char* pnx_last_error(void) { return g_err.c_str(); }
+int pnx_version(void) { return 1; }
+
+int pnx_context_create(int device, pnx_context** out) {
+  if (!out) return fail(PNX_ERR_ARG, "null output pointer");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(PNX_ERR_ARG, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  auto* c = new pnx_context;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(PNX_ERR_DEVICE, "hipStreamCreate failed");
+  }
+  *out = c;
+  return PNX_OK;
+}
+
+int pnx_context_destroy(pnx_context* ctx) {
+  if (!ctx) return PNX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PNX_OK;
+}
+
+int pnx_context_synchronize(pnx_context* ctx) {
+  if (!ctx) return fail(PNX_ERR_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PNX_OK;
+}
+
+int pnx_coefficients(int N, int adjoint, double* out, int capacity) {
+  if (N < 0 || N > kMaxOrder) return fail(PNX_ERR_ARG, "order out of range");
+  std::vector<double> c = build_coefficients(N, adjoint != 0);
+  if (!out) return (int)c.size();
+  if (capacity < (int)c.size()) return fail(PNX_ERR_ARG, "capacity too small");
+  std::copy(c.begin(), c.end(), out);
+  return (int)c.size();
+}
+
+int pnx_advection_block(int N, int c, int d, double* out) {
+  if (N < 0 || N > kMaxOrder || c < 0 || c > 7 || d < 0 || d > 2 || !out) return fail(PNX_ERR_ARG, "bad argument");
+  const Tables t = make_tables(N);
+  const int f = switch_axis(c, d);
+  for (int i = 0; i < t.n[c]; ++i)
+    for (int j = 0; j < t.n[f]; ++j)
+      out[i * t.n[f] + j] = advection_coefficient(t.l[c][i], t.k[c][i], t.l[f][j], t.k[f][j], d);
+  return PNX_OK;
+}
+
+int pnx_boundary_block(int N, int c, int d, int adjoint, double* out) {
+  if (N < 0 || N > kMaxOrder || c < 0 || c > 7 || d < 0 || d > 2 || !out || !odd_in(c, d))
+    return fail(PNX_ERR_ARG, "bad argument");
+  const Tables t = make_tables(N);
+  std::vector<double> co = build_coefficients(N, adjoint != 0);
+  const int n = t.n[c] * t.n[switch_axis(c, d)];
+  std::copy(co.begin() + t.b_off[c][d], co.begin() + t.b_off[c][d] + n, out);
+  return PNX_OK;
+}
+
+int pnx_problem_create(pnx_context* ctx, const pnx_desc* desc, pnx_problem** out) {
+  if (!ctx || !desc || !out) return fail(PNX_ERR_ARG, "null argument");
+  const pnx_desc& d = *desc;
+  if (!order_supported(d.N)) return fail(PNX_ERR_UNSUPPORTED, "PN order must be one of 1, 3, 5, 7, 9");
+  if (d.n_elements < 1 || d.n_elements > 8) return fail(PNX_ERR_ARG, "n_elements must be 1..8");
+  if (d.n_lines < 1 || d.n_lines > 8) return fail(PNX_ERR_ARG, "n_lines must be 1..8");
+  if (d.n_steps < 1 || d.n_beams < 1) return fail(PNX_ERR_ARG, "n_steps and n_beams must be positive");
+  if (!(d.E_initial > d.E_cutoff)) return fail(PNX_ERR_ARG, "E_initial must exceed E_cutoff");
+  int dims = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (d.M[a] < 1) return fail(PNX_ERR_ARG, "grid sizes must be >= 1");
+    if (d.M[a] == 1) continue;
+    if (d.M[a] < 2 || a != dims) return fail(PNX_ERR_ARG, "non-singleton axes must come first (x, then y, then z)");
+    if (!(d.hi[a] > d.lo[a])) return fail(PNX_ERR_ARG, "axis extent must be positive");
+    ++dims;
+  }
+  if (dims == 0) return fail(PNX_ERR_ARG, "at least the x axis must be resolved");
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto* p = new pnx_problem;
+  p->ctx = ctx;
+  p->d = d;
+  p->T = make_tables(d.N);
+  Layout& L = p->L;
+  L.Mx = d.M[0];
+  L.My = d.M[1];
+  L.Mz = d.M[2];
+  L.dims = dims;
+  L.N = d.N;
+  L.n_e = d.n_elements;
+  L.n_lines = d.n_lines;
+  // x pitch padded to 16 doubles (128 B cache lines) once rows are long enough
+  const int xneed = L.Mx + 1;
+  L.X = xneed >= 64 ? (xneed + 15) / 16 * 16 : xneed;
+  L.Y = L.My == 1 ? 1 : L.My + 1;
+  L.Z = L.Mz == 1 ? 1 : L.Mz + 1;
+  L.V = (long)L.X * L.Y * L.Z;
+  long off = 0;
+  for (int c = 0; c < 8; ++c) {
+    if (class_active(c, dims) && p->T.n[c] > 0) {
+      L.cls_off[c] = off;
+      off += p->T.n[c] * L.V;
+    } else {
+      L.cls_off[c] = 0;  // never touched by the kernels
+    }
+  }
+  L.beam_stride = off;
+  double h[3];
+  for (int a = 0; a < 3; ++a) {
+    h[a] = d.M[a] == 1 ? 1.0 : (d.hi[a] - d.lo[a]) / (d.M[a] - 1);
+    L.inv_h[a] = 1.0 / h[a];
+    for (int c = 0; c < 8; ++c) p->class_n[c][a] = d.M[a] == 1 ? 1 : (odd_in(c, a) ? d.M[a] + 1 : d.M[a]);
+  }
+  L.dvol = h[0] * h[1] * h[2];
+  L.n_cells = L.Mx * L.My * L.Mz;
+  L.n_tiles = (L.n_cells + kBlock - 1) / kBlock;
+  p->rec_len = d.n_elements + d.n_elements * (d.N + 1) + 1 + d.n_lines;
+  L.rec_len = p->rec_len;
+  p->sigma_off = d.n_elements + d.n_elements * (d.N + 1) + 1;
+  p->dE = (d.E_initial - d.E_cutoff) / d.n_steps;
+  p->K = d.ckpt_every > 0 ? std::min(d.ckpt_every, d.n_steps) : d.n_steps;
+  p->nck = (d.n_steps + p->K - 1) / p->K;
+  long ro = 0, rbo = 0;
+  for (int c = 0; c < 8; ++c) {
+    const bool act = class_active(c, dims) && p->T.n[c] > 0;
+    p->rho_off[c] = act ? ro : 0;
+    p->rb_off[c] = act ? rbo : 0;
+    if (act) {
+      ro += (long)d.n_elements * L.V;
+      rbo += (long)d.n_elements * L.V;
+    }
+  }
+  p->rho_total = ro;
+  p->rb_stride = rbo;
+  p->prof_stride = 8L * L.Y * L.Z;
+  p->partial_step = (long)L.n_tiles * d.n_lines;
+  p->partial_beam = (long)(d.n_steps + 1) * p->partial_step;
+  int rc = 0;
+  if ((rc = p->state.alloc((size_t)L.beam_stride * d.n_beams)) || (rc = p->rho.alloc(ro)) ||
+      (rc = p->W.alloc((size_t)d.n_lines * L.V)) || (rc = p->prof.alloc((size_t)p->prof_stride * d.n_beams)) ||
+      (rc = p->mms.alloc(8 * 16)) || (rc = p->partial.alloc((size_t)p->partial_beam * d.n_beams)) ||
+      (rc = p->I_dev.alloc((size_t)d.n_beams * d.n_lines)) ||
+      (rc = p->rec_f.alloc((size_t)(d.n_steps + 1) * p->rec_len))) {
+    delete p;
+    return rc;
+  }
+  std::vector<double> cf = build_coefficients(d.N, false), ca = build_coefficients(d.N, true);
+  if ((rc = p->coef_f.alloc(cf.size())) || (rc = p->coef_a.alloc(ca.size()))) {
+    delete p;
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  if (hipMemcpyAsync(p->coef_f.p, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(p->coef_a.p, ca.data(), ca.size() * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+      launch_fill(p->state.p, (long)p->state.n, 0.0, s) != hipSuccess ||
+      launch_fill(p->rho.p, (long)p->rho.n, 0.0, s) != hipSuccess ||
+      launch_fill(p->W.p, (long)p->W.n, 0.0, s) != hipSuccess ||
+      launch_fill(p->prof.p, (long)p->prof.n, 0.0, s) != hipSuccess ||
+      launch_fill(p->mms.p, (long)p->mms.n, 0.0, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    delete p;
+    return fail(PNX_ERR_DEVICE, "problem initialisation failed");
+  }
+  *out = p;
+  return PNX_OK;
+}
+
+int pnx_problem_destroy(pnx_problem* p) {
+  if (!p) return PNX_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  destroy_graph(p);
+  delete p;
+  return PNX_OK;
+}
+
+int64_t pnx_problem_device_bytes(const pnx_problem* p) {
+  if (!p) return 0;
+  size_t n = 0;
+  for (const DevBuf* b : {&p->state, &p->rho, &p->W, &p->coef_f, &p->coef_a, &p->rec_f, &p->rec_a, &p->rec_d,
+                          &p->prof, &p->mms, &p->partial, &p->I_dev, &p->lam, &p->ckpt, &p->seg, &p->rho_bar_b,
+                          &p->gen_b, &p->ibar, &p->sum_buf})
+    n += b->n;
+  return (int64_t)(n * sizeof(double));
+}
+
+int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj, const double* dens) {
+  if (!p || !fwd) return fail(PNX_ERR_ARG, "null argument");
+  if ((adj == nullptr) != (dens == nullptr)) return fail(PNX_ERR_ARG, "adj and dens must be given together");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const size_t n = (size_t)(p->d.n_steps + 1) * p->rec_len;
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->rec_f.p, fwd, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (adj) {
+    int rc;
+    if ((rc = p->rec_a.alloc(n)) || (rc = p->rec_d.alloc(n))) return rc;
+    HIP_TRY(hipMemcpyAsync(p->rec_a.p, adj, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(p->rec_d.p, dens, n * sizeof(double), hipMemcpyHostToDevice, s));
+    p->have_adj = true;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  p->have_rec = true;
+  return PNX_OK;
+}
+
+int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const double* W) {
+  if (!p || !rho || !W) return fail(PNX_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const Layout& L = p->L;
+  const int ne = p->d.n_elements;
+  std::vector<double> h((size_t)p->rho_total, 0.0);
+  for (int c = 0; c < 8; ++c) {
+    if (!(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    if (!rho[c]) return fail(PNX_ERR_ARG, "densities missing for an updated class");
+    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2];
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long q = i + (long)nx * (j + (long)ny * k);
+          const long pd = i + (long)L.X * (j + (long)L.Y * k);
+          for (int e = 0; e < ne; ++e) h[p->rho_off[c] + e * L.V + pd] = rho[c][q * ne + e];
+        }
+  }
+  std::vector<double> w((size_t)p->d.n_lines * L.V, 0.0);
+  const int nl = p->d.n_lines;
+  for (int k = 0; k < L.Mz; ++k)
+    for (int j = 0; j < L.My; ++j)
+      for (int i = 0; i < L.Mx; ++i) {
+        const long q = i + (long)L.Mx * (j + (long)L.My * k);
+        const long pd = i + (long)L.X * (j + (long)L.Y * k);
+        for (int r = 0; r < nl; ++r) w[r * L.V + pd] = W[q * nl + r];
+      }
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->rho.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->W.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  p->have_mat = true;
+  return PNX_OK;
+}
+
+int pnx_problem_set_beams(pnx_problem* p, const double* const mms[8], const double* const* ppos) {
+  if (!p || !mms || !ppos) return fail(PNX_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const Layout& L = p->L;
+  std::vector<double> m(8 * 16, 0.0), pr((size_t)p->prof_stride * p->d.n_beams, 0.0);
+  for (int c = 0; c < 8; ++c) {
+    if (!odd_in(c, 0) || !(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    if (!mms[c]) return fail(PNX_ERR_ARG, "beam moments missing for an x-odd class");
+    for (int i = 0; i < p->T.n[c]; ++i) m[c * 16 + i] = mms[c][i];
+    const int ny = p->class_n[c][1], nz = p->class_n[c][2];
+    for (int b = 0; b < p->d.n_beams; ++b) {
+      const double* src = ppos[b * 8 + c];
+      if (!src) return fail(PNX_ERR_ARG, "beam profile missing for an x-odd class");
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) pr[b * p->prof_stride + c * (long)L.Y * L.Z + j + (long)L.Y * k] = src[j + ny * k];
+    }
+  }
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->mms.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->prof.p, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  p->have_beams = true;
+  return PNX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+HalfStepArgs half_args(pnx_problem* p, int mode, const double* rec, double dt, const double* from, const double* src,
+                       double* dst, double* partial) {
+  HalfStepArgs a{};
+  a.L = p->L;
+  a.from = from;
+  a.to_src = src;
+  a.to_dst = dst;
+  a.rho = p->rho.p;
+  for (int c = 0; c < 8; ++c) a.rho_off[c] = p->rho_off[c];
+  a.coef = mode == kForward ? p->coef_f.p : p->coef_a.p;
+  a.rec = rec;
+  a.prof = p->prof.p;
+  a.prof_stride = p->prof_stride;
+  a.mms = p->mms.p;
+  a.W = p->W.p;
+  a.ibar = mode == kAdjoint ? p->ibar.p : nullptr;
+  a.partial = partial;
+  a.partial_stride = p->partial_beam;
+  a.dt = dt;
+  a.n_beams = p->d.n_beams;
+  return a;
+}
+
+// One energy step of the forward problem (step_pde!): odd half, even full, odd half.
+// src -> dst (dst may equal src); partial receives the intensity partials of this step.
+int forward_step(pnx_problem* p, int k, const double* src, double* dst, double* partial) {
+  hipStream_t s = p->ctx->stream;
+  const double* rec = p->rec_f.p + (long)k * p->rec_len;
+  const int N = p->d.N, dims = p->L.dims;
+  const double h = p->dE / 2;
+  HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, src, src, dst, nullptr), s));
+  HIP_TRY(launch_half_step(N, dims, 0, kForward, half_args(p, kForward, rec, p->dE, dst, src, dst, partial), s));
+  HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, dst, dst, dst, nullptr), s));
+  return PNX_OK;
+}
+
+int adjoint_step(pnx_problem* p, int i, double* lam) {
+  hipStream_t s = p->ctx->stream;
+  const double* rec = p->rec_a.p + (long)i * p->rec_len;
+  const int N = p->d.N, dims = p->L.dims;
+  const double h = p->dE / 2;
+  HIP_TRY(launch_half_step(N, dims, 1, kAdjoint, half_args(p, kAdjoint, rec, h, lam, lam, lam, nullptr), s));
+  HIP_TRY(launch_half_step(N, dims, 0, kAdjoint, half_args(p, kAdjoint, rec, p->dE, lam, lam, lam, nullptr), s));
+  HIP_TRY(launch_half_step(N, dims, 1, kAdjoint, half_args(p, kAdjoint, rec, h, lam, lam, lam, nullptr), s));
+  return PNX_OK;
+}
+
+int enqueue_forward(pnx_problem* p) {
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(launch_fill(p->state.p, (long)p->L.beam_stride * p->d.n_beams, 0.0, s));
+  for (int k = 1; k <= p->d.n_steps; ++k) {
+    int rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->partial_step);
+    if (rc) return rc;
+  }
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->partial_step, p->partial_beam, p->L.n_tiles, p->rec_f.p, p->rec_len,
+                                  p->sigma_off, p->d.n_steps, p->d.n_lines, p->dE, p->L.dvol, p->d.n_beams,
+                                  p->I_dev.p, s));
+  return PNX_OK;
+}
+
+int ensure_graph(pnx_problem* p) {
+  if (p->fwd_graph) return PNX_OK;
+  hipStream_t s = p->ctx->stream;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_forward(p);
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PNX_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&p->fwd_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    p->fwd_graph = nullptr;
+    return fail(PNX_ERR_DEVICE, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+  }
+  return PNX_OK;
+}
+
+int check_ready(pnx_problem* p, bool adjoint) {
+  if (!p) return fail(PNX_ERR_ARG, "null problem");
+  if (!p->have_rec || !p->have_mat || !p->have_beams)
+    return fail(PNX_ERR_STATE, "records, material and beams must be set before solving");
+  if (adjoint && !p->have_adj) return fail(PNX_ERR_STATE, "adjoint records were not set");
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return fail(PNX_ERR_DEVICE, "hipSetDevice failed");
+  return PNX_OK;
+}
+
+int alloc_adjoint(pnx_problem* p) {
+  const size_t st = (size_t)p->L.beam_stride * p->d.n_beams;
+  int rc;
+  if ((rc = p->lam.alloc(st)) || (rc = p->ckpt.alloc(st * p->nck)) || (rc = p->seg.alloc(st * p->K)) ||
+      (rc = p->rho_bar_b.alloc((size_t)p->rb_stride * p->d.n_beams)) ||
+      (rc = p->gen_b.alloc((size_t)p->d.n_lines * p->L.V * p->d.n_beams)) ||
+      (rc = p->ibar.alloc((size_t)p->d.n_beams * p->d.n_lines)) ||
+      (rc = p->sum_buf.alloc(std::max((size_t)p->rb_stride, (size_t)p->d.n_lines * p->L.V))))
+    return rc;
+  return PNX_OK;
+}
+
+DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
+  DensArgs a{};
+  a.L = p->L;
+  a.u = u;
+  a.lam = p->lam.p;
+  a.rho = p->rho.p;
+  for (int c = 0; c < 8; ++c) {
+    a.rho_off[c] = p->rho_off[c];
+    a.rb_off[c] = p->rb_off[c];
+  }
+  a.coef = p->coef_f.p;
+  a.rec = p->rec_d.p + (long)i * p->rec_len;
+  a.prof = p->prof.p;
+  a.prof_stride = p->prof_stride;
+  a.mms = p->mms.p;
+  a.W = p->W.p;
+  a.ibar = p->ibar.p;
+  a.rho_bar = p->rho_bar_b.p;
+  a.rb_stride = p->rb_stride;
+  a.gen = p->gen_b.p;
+  a.gen_stride = (long)p->d.n_lines * p->L.V;
+  a.dE = p->dE;
+  a.int_fac = int_fac;
+  a.n_beams = p->d.n_beams;
+  return a;
+}
+
+// Forward pass with checkpoints, then the reverse sweep of the reference
+// pullback (pn_intensities.jl:381-404).
+int enqueue_forward_adjoint(pnx_problem* p) {
+  hipStream_t s = p->ctx->stream;
+  const long st = p->L.beam_stride * p->d.n_beams;
+  const int n = p->d.n_steps, K = p->K, N = p->d.N, dims = p->L.dims;
+  int rc;
+  HIP_TRY(launch_fill(p->state.p, st, 0.0, s));
+  for (int k = 1; k <= n; ++k) {
+    if ((k - 1) % K == 0)
+      HIP_TRY(hipMemcpyAsync(p->ckpt.p + (long)((k - 1) / K) * st, p->state.p, st * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->partial_step))) return rc;
+  }
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->partial_step, p->partial_beam, p->L.n_tiles, p->rec_f.p, p->rec_len,
+                                  p->sigma_off, n, p->d.n_lines, p->dE, p->L.dvol, p->d.n_beams, p->I_dev.p, s));
+  HIP_TRY(launch_fill(p->lam.p, st, 0.0, s));
+  HIP_TRY(launch_fill(p->rho_bar_b.p, (long)p->rho_bar_b.n, 0.0, s));
+  HIP_TRY(launch_fill(p->gen_b.p, (long)p->gen_b.n, 0.0, s));
+  for (int sgi = p->nck - 1; sgi >= 0; --sgi) {
+    const int k0 = sgi * K, k1 = std::min(n, k0 + K);
+    // recompute u_{k0+1} .. u_{k1} into the segment buffer
+    const double* src = p->ckpt.p + (long)sgi * st;
+    for (int k = k0 + 1; k <= k1; ++k) {
+      double* dst = p->seg.p + (long)(k - k0 - 1) * st;
+      if ((rc = forward_step(p, k, src, dst, p->partial.p))) return rc;  // partials of step 0 slot are scratch
+      src = dst;
+    }
+    for (int i = k1; i > k0; --i) {
+      const double* u = p->seg.p + (long)(i - k0 - 1) * st;
+      HIP_TRY(launch_dens(N, dims, dens_args(p, i, u, 1.0), s));
+      if ((rc = adjoint_step(p, i, p->lam.p))) return rc;
+    }
+  }
+  // final accumulation at u_0 = 0 with weight 0.5 (pn_intensities.jl:402-404)
+  HIP_TRY(launch_dens(N, dims, dens_args(p, 0, p->ckpt.p, 0.5), s));
+  return PNX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pnx_forward(pnx_problem* p, double* I_out) {
+  int rc = check_ready(p, false);
+  if (rc) return rc;
+  if (!I_out) return fail(PNX_ERR_ARG, "null output");
+  if ((rc = ensure_graph(p))) return rc;
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipGraphLaunch(p->fwd_graph, s));
+  HIP_TRY(hipMemcpyAsync(I_out, p->I_dev.p, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return PNX_OK;
+}
+
+int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out) {
+  int rc = check_ready(p, true);
+  if (rc) return rc;
+  if (!I_bar || !I_out) return fail(PNX_ERR_ARG, "null argument");
+  if ((rc = alloc_adjoint(p))) return rc;
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->ibar.p, I_bar, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyHostToDevice, s));
+  if ((rc = enqueue_forward_adjoint(p))) return rc;
+  HIP_TRY(hipMemcpyAsync(I_out, p->I_dev.p, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
+  const Layout& L = p->L;
+  const int ne = p->d.n_elements;
+  if (rho_bar) {
+    HIP_TRY(launch_beam_sum(p->rho_bar_b.p, p->rb_stride, p->d.n_beams, p->rb_stride, p->sum_buf.p, s));
+    std::vector<double> h((size_t)p->rb_stride);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->sum_buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int c = 0; c < 8; ++c) {
+      if (!rho_bar[c]) continue;
+      const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2];
+      const bool act = class_active(c, L.dims) && p->T.n[c] > 0;
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+          for (int i = 0; i < nx; ++i) {
+            const long q = i + (long)nx * (j + (long)ny * k);
+            const long pd = i + (long)L.X * (j + (long)L.Y * k);
+            for (int e = 0; e < ne; ++e) rho_bar[c][q * ne + e] = act ? h[p->rb_off[c] + e * L.V + pd] : 0.0;
+          }
+    }
+  }
+  if (gen_out) {
+    const long gs = (long)p->d.n_lines * L.V;
+    HIP_TRY(launch_beam_sum(p->gen_b.p, gs, p->d.n_beams, gs, p->sum_buf.p, s));
+    std::vector<double> h((size_t)gs);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->sum_buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int nl = p->d.n_lines;
+    for (int k = 0; k < L.Mz; ++k)
+      for (int j = 0; j < L.My; ++j)
+        for (int i = 0; i < L.Mx; ++i) {
+          const long q = i + (long)L.Mx * (j + (long)L.My * k);
+          const long pd = i + (long)L.X * (j + (long)L.Y * k);
+          for (int r = 0; r < nl; ++r) gen_out[q * nl + r] = h[r * L.V + pd];
+        }
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return PNX_OK;
+}
+
+int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_bar, double* const u[8]) {
+  int rc = check_ready(p, adjoint != 0);
+  if (rc) return rc;
+  if (!u || beam < 0 || beam >= p->d.n_beams || record < 0 || record > p->d.n_steps)
+    return fail(PNX_ERR_ARG, "bad argument");
+  const Layout& L = p->L;
+  const int dims = L.dims;
+  hipStream_t s = p->ctx->stream;
+  std::vector<double> h((size_t)L.beam_stride, 0.0);
+  for (int c = 0; c < 8; ++c) {
+    if (!(class_active(c, dims) && p->T.n[c] > 0)) continue;
+    if (!u[c]) return fail(PNX_ERR_ARG, "state missing for an updated class");
+    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2], nm = p->T.n[c];
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long q = i + (long)nx * (j + (long)ny * k);
+          const long pd = i + (long)L.X * (j + (long)L.Y * k);
+          for (int m = 0; m < nm; ++m) h[L.cls_off[c] + m * L.V + pd] = u[c][q * nm + m];
+        }
+  }
+  double* sb = p->state.p + (long)beam * L.beam_stride;
+  HIP_TRY(hipMemcpyAsync(sb, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  if (adjoint) {
+    if ((rc = alloc_adjoint(p))) return rc;
+    std::vector<double> ib((size_t)p->d.n_beams * p->d.n_lines, 0.0);
+    if (I_bar)
+      for (int r = 0; r < p->d.n_lines; ++r) ib[beam * p->d.n_lines + r] = I_bar[r];
+    HIP_TRY(hipMemcpyAsync(p->ibar.p, ib.data(), ib.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    if ((rc = adjoint_step(p, record, p->state.p))) return rc;
+  } else {
+    if ((rc = forward_step(p, record, p->state.p, p->state.p, p->partial.p))) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(h.data(), sb, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int c = 0; c < 8; ++c) {
+    if (!(class_active(c, dims) && p->T.n[c] > 0) || !u[c]) continue;
+    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2], nm = p->T.n[c];
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long q = i + (long)nx * (j + (long)ny * k);
+          const long pd = i + (long)L.X * (j + (long)L.Y * k);
+          for (int m = 0; m < nm; ++m) u[c][q * nm + m] = h[L.cls_off[c] + m * L.V + pd];
+        }
+  }
+  return PNX_OK;
+}
+
+int pnx_forward_timed(pnx_problem* p, int reps, int use_graph, double* ms_out) {
+  int rc = check_ready(p, false);
+  if (rc) return rc;
+  if (reps < 1 || !ms_out) return fail(PNX_ERR_ARG, "bad argument");
+  if (use_graph && (rc = ensure_graph(p))) return rc;
+  hipStream_t s = p->ctx->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) {
+    if (use_graph) {
+      HIP_TRY(hipGraphLaunch(p->fwd_graph, s));
+    } else if ((rc = enqueue_forward(p))) {
+      return rc;
+    }
+  }
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms / reps;
+  return PNX_OK;
+}
+
+int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out) {
+  int rc = check_ready(p, true);
+  if (rc) return rc;
+  if (reps < 1 || !ms_out) return fail(PNX_ERR_ARG, "bad argument");
+  if ((rc = alloc_adjoint(p))) return rc;
+  hipStream_t s = p->ctx->stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = enqueue_forward_adjoint(p))) return rc;
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms / reps;
+  return PNX_OK;
+}
+
+int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, double* bytes_out) {
+  int rc = check_ready(p, false);
+  if (rc) return rc;
+  if (reps < 1 || (parity != 0 && parity != 1)) return fail(PNX_ERR_ARG, "bad argument");
+  const Layout& L = p->L;
+  // algorithmic bytes: read+write of the updated classes' interior, one read of
+  // every non-ghost point of the other parity, the densities of the updated
+  // points, and (even step) the line weights at eee points
+  double upd = 0, src = 0, pts = 0;
+  for (int c = 0; c < 8; ++c) {
+    if (!(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    double np = 1;
+    for (int a = 0; a < L.dims; ++a) np *= odd_in(c, a) ? p->d.M[a] - 1 : p->d.M[a];
+    if (parity_of(c) == parity) {
+      upd += np * p->T.n[c] * 8.0;
+      pts += np;
+    } else {
+      src += np * p->T.n[c] * 8.0;
+    }
+  }
+  double bytes = 2 * upd + src + 8.0 * p->d.n_elements * pts;
+  if (parity == 0) bytes += 8.0 * L.n_cells * p->d.n_lines;
+  bytes *= p->d.n_beams;
+  if (bytes_out) *bytes_out = bytes;
+  if (!ms_out) return PNX_OK;
+  hipStream_t s = p->ctx->stream;
+  const double* rec = p->rec_f.p + p->rec_len;
+  const double dt = parity == 0 ? p->dE : p->dE / 2;
+  HalfStepArgs a = half_args(p, kForward, rec, dt, p->state.p, p->state.p, p->state.p,
+                             parity == 0 ? p->partial.p + p->partial_step : nullptr);
+  HIP_TRY(launch_half_step(p->d.N, L.dims, parity, kForward, a, s));  // warm-up
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) HIP_TRY(launch_half_step(p->d.N, L.dims, parity, kForward, a, s));
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms / reps;
+  return PNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Launch dispatch and the small helper kernels (fill, beam sums, intensity reduction).
+#include <hip/hip_runtime.h>
+
+#include "pnx_kernels.hpp"
+
+namespace pnx {
+
+constexpr int kMaxLinesR = 8;
+
+hipError_t half_step_n1(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t dens_n1(int dims, const DensArgs& a, hipStream_t s);
+hipError_t half_step_n3(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t dens_n3(int dims, const DensArgs& a, hipStream_t s);
+hipError_t half_step_n5(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t dens_n5(int dims, const DensArgs& a, hipStream_t s);
+hipError_t half_step_n7(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t dens_n7(int dims, const DensArgs& a, hipStream_t s);
+hipError_t half_step_n9(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t dens_n9(int dims, const DensArgs& a, hipStream_t s);
+
+hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return half_step_n1(dims, parity, mode, a, s);
+    case 3: return half_step_n3(dims, parity, mode, a, s);
+    case 5: return half_step_n5(dims, parity, mode, a, s);
+    case 7: return half_step_n7(dims, parity, mode, a, s);
+    case 9: return half_step_n9(dims, parity, mode, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return dens_n1(dims, a, s);
+    case 3: return dens_n3(dims, a, s);
+    case 5: return dens_n5(dims, a, s);
+    case 7: return dens_n7(dims, a, s);
+    case 9: return dens_n9(dims, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+__device__ __forceinline__ double wave_sum_r(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void fill_kernel(double* __restrict__ p, long n, double v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long x = i; x < n; x += stride) p[x] = v;
+}
+
+__global__ void beam_sum_kernel(const double* __restrict__ in, long stride, int n_beams, long n, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int b = 0; b < n_beams; ++b) s += in[b * stride + i];
+  out[i] = s;
+}
+
+// One block per beam; thread t sums the steps k = 1 + t, 1 + t + 256, ...
+__global__ __launch_bounds__(kBlock) void reduce_intensity_kernel(const double* __restrict__ partial, long step_stride,
+                                                                  long beam_stride, int n_tiles,
+                                                                  const double* __restrict__ rec, int rec_len,
+                                                                  int sigma_off, int n_steps, int n_lines, double dE,
+                                                                  double dvol, double* __restrict__ I_out) {
+  const int b = blockIdx.x;
+  __shared__ double red[kBlock / 64][kMaxLinesR];
+  double acc[kMaxLinesR];
+#pragma unroll
+  for (int r = 0; r < kMaxLinesR; ++r) acc[r] = 0.0;
+  for (int kk = 1 + threadIdx.x; kk <= n_steps; kk += kBlock) {
+    const double* __restrict__ ps = partial + b * beam_stride + kk * step_stride;
+    const double* __restrict__ sig = rec + (long)kk * rec_len + sigma_off;
+    for (int r = 0; r < n_lines && r < kMaxLinesR; ++r) {
+      double integral = 0.0;
+      for (int t = 0; t < n_tiles; ++t) integral += ps[(long)t * n_lines + r];
+      acc[r] += dE * dvol * sig[r] * integral * 1.0;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = 0; r < n_lines && r < kMaxLinesR; ++r) {
+    const double s = wave_sum_r(acc[r]);
+    if (lane == 0) red[wave][r] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_lines && threadIdx.x < kMaxLinesR) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    I_out[b * n_lines + threadIdx.x] = s;
+  }
+}
+
+hipError_t launch_fill(double* p, long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_beam_sum(const double* in, long stride, int n_beams, long n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long blocks = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(beam_sum_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, in, stride, n_beams, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,
+                                   const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
+                                   double dE, double dvol, int n_beams, double* I_out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_intensity_kernel, dim3(n_beams), dim3(kBlock), 0, s, partial, step_stride, beam_stride,
+                     n_tiles, fwd_rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol, I_out);
+  return hipGetLastError();
+}
+
+}  // namespace pnx

@@ -1,0 +1,86 @@
+// Kernel argument blocks and launch entry points of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pnx {
+
+enum Mode : int { kForward = 0, kAdjoint = 1 };
+
+// Device layout of one batch.  Every class array uses the same padded extents
+// X x Y x Z (X = x pitch >= Mx+1, Y = My+1 or 1, Z = Mz+1 or 1) so that a
+// point (i, j, k) has the same linear index p = i + X*(j + Y*k) in every
+// class; class c holds n_c moment planes of V = X*Y*Z doubles.  The state of
+// beam b starts at state + b*beam_stride, class c at offset cls_off[c].
+struct Layout {
+  int Mx, My, Mz;   // grid points (1 = singleton)
+  int X, Y, Z;      // padded extents
+  long V;           // doubles per moment plane
+  int dims;         // number of non-singleton axes
+  int n_e, n_lines, N;
+  long cls_off[8];  // class offsets within one beam state (-1 = inactive)
+  long beam_stride; // doubles per beam state
+  double inv_h[3];
+  double dvol;
+  int n_cells;      // Mx*My*Mz unit cells
+  int n_tiles;      // ceil(n_cells / kBlock)
+  int rec_len;
+};
+
+constexpr int kBlock = 256;
+
+struct HalfStepArgs {
+  Layout L;
+  const double* from;     // state read for the other parity (beam 0)
+  const double* to_src;   // state read for the updated parity (beam 0)
+  double* to_dst;         // state written for the updated parity (beam 0)
+  const double* rho;      // [c][e][V] (class offsets rho_off)
+  long rho_off[8];
+  const double* coef;     // advection + boundary values for this order and direction
+  const double* rec;      // energy-point record of this step
+  const double* prof;     // [b][c][Y*Z] beam profiles of x-odd classes (forward)
+  long prof_stride;       // doubles per beam in prof
+  const double* mms;      // [c][16] beam moments
+  const double* W;        // [r][V] line weights at eee points
+  const double* ibar;     // [b][n_lines] adjoint seed (adjoint eee source), may be null
+  double* partial;        // [b][n_tiles][n_lines] intensity partial sums of this step (forward, even)
+  long partial_stride;    // doubles per beam in partial
+  double dt;
+  int n_beams;
+};
+
+struct DensArgs {
+  Layout L;
+  const double* u;        // taped forward state (beam 0)
+  const double* lam;      // adjoint state (beam 0)
+  const double* rho;
+  long rho_off[8];
+  const double* coef;     // forward coefficients
+  const double* rec;      // dens record of this step
+  const double* prof;
+  long prof_stride;
+  const double* mms;
+  const double* W;
+  const double* ibar;     // [b][n_lines]
+  double* rho_bar;        // [b][c][e][V]
+  long rb_stride;         // doubles per beam
+  long rb_off[8];
+  double* gen;            // [b][r][V] generation-weighted pullback
+  long gen_stride;
+  double dE;
+  double int_fac;
+  int n_beams;
+};
+
+// Launchers (defined in pnx_kernels.hip); return hipSuccess or the launch error.
+hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s);
+hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
+// I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps
+hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,
+                                   const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
+                                   double dE, double dvol, int n_beams, double* I_out, hipStream_t s);
+// out[i] = sum_b in[b*stride + i], i < n
+hipError_t launch_beam_sum(const double* in, long stride, int n_beams, long n, double* out, hipStream_t s);
+
+}  // namespace pnx

@@ -1,0 +1,361 @@
+// gfx950 kernels of the PN-CSD staggered energy step (templates; instantiated per
+// PN order in pnx_kernels_n<N>.hip).
+//
+// One fused kernel per half step (starmap/src/solver.jl:171-196): a thread owns
+// one unit cell (i, j, k) of the staggered grid and updates every class of the
+// stepped parity at that cell.  Per class it
+//   * applies the sparse advection blocks as two-point differences along each
+//     non-singleton axis (kernel!/matmuldiff!, starmap/src/solver_core.jl:59-137),
+//     with the CSC traversal order of the reference: column by column;
+//   * rebuilds the boundary ghost value of a neighbour in registers when the
+//     difference reaches past the domain (apply_boundary_source!/_zero!,
+//     starmap/src/boundary_conditions.jl:67-102): ghosts are functions of the
+//     neighbour's first interior value and of this cell's own values, so they
+//     are never stored;
+//   * forms S and C from the per-point partial densities and the energy record
+//     (weighted_sum!, pn_solver/pn_problem.jl:213-249) instead of reading
+//     precomputed S, C arrays;
+//   * applies the exponential integrator factor once per degree
+//     (right_hand_side!/expm1div, starmap/src/solver_core.jl:38-106) and the
+//     update u += dt du (solver.jl:122-124);
+//   * in the forward even step, accumulates the eee zeroth moment weighted by
+//     the line weights into per-block partial sums (integrate_step_and_add!,
+//     pn_solver/x_ray_intensities.jl:172-191).
+// The state is moment-major: each moment of each class is a plane of V
+// doubles with x fastest, so a wavefront reads 64 consecutive doubles per
+// moment and the stencil neighbours along x come from the same cache lines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pnx_kernels.hpp"
+#include "pnx_tables.hpp"
+
+namespace pnx {
+
+constexpr int kMaxLines = 8;
+constexpr int kMmsStride = 16;
+
+template <int N, int C>
+constexpr int ncls() {
+  return Tab<N>::t.n[C];
+}
+
+template <int N, int C>
+constexpr bool class_has_degree(int l) {
+  for (int i = 0; i < Tab<N>::t.n[C]; ++i)
+    if (Tab<N>::t.l[C][i] == l) return true;
+  return false;
+}
+
+__device__ __forceinline__ double expm1div(double c) {
+  if (fabs(c) <= 0.0002) return 1.0 + 0.5 * c + 1.0 / 6.0 * c * c;
+  return (exp(c) - 1.0) / c;
+}
+
+// du += sum over non-singleton axes of A_d[C] (u_F[R] - u_F[L]) / h_d, with
+// boundary ghosts of F rebuilt from this point's own values u (length n_C).
+template <int N, int DIM, int C, bool SRC>
+__device__ __forceinline__ void stencil(const Layout& L, const double* __restrict__ from,
+                                        const double* __restrict__ coef, const double* u, double* du, int i, int j,
+                                        int k, long p, double gfac, const double* __restrict__ prof_b,
+                                        const double* __restrict__ mms) {
+  constexpr int NC = ncls<N, C>();
+  const long V = L.V;
+  static_for<0, DIM>([&](auto Dc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr int F = switch_axis(C, D);
+    constexpr int NF = ncls<N, F>();
+    if constexpr (NF > 0) {
+      constexpr bool codd = odd_in(C, D) != 0;
+      const long st = D == 0 ? 1L : (D == 1 ? (long)L.X : (long)L.X * L.Y);
+      const long pL = codd ? p - st : p;
+      const long pR = codd ? p : p + st;
+      const double* __restrict__ uf = from + L.cls_off[F];
+      double vL[NF], vR[NF];
+#pragma unroll
+      for (int J = 0; J < NF; ++J) {
+        vL[J] = uf[J * V + pL];
+        vR[J] = uf[J * V + pR];
+      }
+      if constexpr (!codd) {
+        const int idx = D == 0 ? i : (D == 1 ? j : k);
+        const int M = D == 0 ? L.Mx : (D == 1 ? L.My : L.Mz);
+        const bool left = idx == 0, right = idx == M - 1;
+        if (left || right) {
+          constexpr int boff = Tab<N>::t.b_off[F][D];
+          const double* __restrict__ Bm = coef + boff;
+          double pr = 0.0;
+          if constexpr (SRC && D == 0) pr = prof_b[F * (long)L.Y * L.Z + j + (long)L.Y * k];
+#pragma unroll
+          for (int J = 0; J < NF; ++J) {
+            double bc = 0.0;
+#pragma unroll
+            for (int I = 0; I < NC; ++I) bc += Bm[J * NC + I] * u[I];
+            if (left) {
+              vL[J] = 2.0 * (-bc + 0.0) - vR[J];
+            } else {
+              double g = 0.0;
+              if constexpr (SRC && D == 0) g = (mms[F * kMmsStride + J] * pr) * gfac;
+              vR[J] = 2.0 * (bc + g) - vL[J];
+            }
+          }
+        }
+      }
+      const double ih = L.inv_h[D];
+      static_for<0, NF>([&](auto Jc) {
+        constexpr int J = decltype(Jc)::value;
+        const double d = (vR[J] - vL[J]) * ih;
+        static_for<0, NC>([&](auto Ic) {
+          constexpr int I = decltype(Ic)::value;
+          constexpr int z = Tab<N>::t.nz[C][D][I][J];
+          if constexpr (z >= 0) du[I] += coef[z] * d;
+        });
+      });
+    }
+  });
+}
+
+// S and the per-degree C of a point (weighted_sum!, pn_problem.jl:213-220).
+template <int N>
+__device__ __forceinline__ void mix(const double* __restrict__ rc, long V, long p, int ne,
+                                    const double* __restrict__ rec, double& S, double* Cl) {
+  S = 0.0;
+#pragma unroll
+  for (int l = 0; l <= N; ++l) Cl[l] = 0.0;
+  for (int e = 0; e < ne; ++e) {
+    const double r = rc[e * V + p];
+    S += r * rec[e];
+    const double* __restrict__ ce = rec + ne + e * (N + 1);
+#pragma unroll
+    for (int l = 0; l <= N; ++l) Cl[l] += r * ce[l];
+  }
+}
+
+template <int N, int DIM, int C, int MODE, int PAR>
+__device__ __forceinline__ void update_class(const HalfStepArgs& a, const double* __restrict__ from,
+                                             const double* src, double* dst, int i, int j, int k, long p, int beam,
+                                             double* contrib) {
+  // src and dst are the same buffer for in-place steps: the own values are
+  // read before they are overwritten, so neither pointer is restrict.
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  const int ne = L.n_e;
+  const double* uc = src + L.cls_off[C];
+  double u[NC], du[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    u[m] = uc[m * V + p];
+    du[m] = 0.0;
+  }
+  const double gfac = a.rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, MODE == kForward>(L, from, a.coef, u, du, i, j, k, p, gfac, a.prof + beam * a.prof_stride,
+                                       a.mms);
+  double S, Cl[N + 1];
+  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
+  double phi[N + 1];
+  static_for<0, N + 1>([&](auto lc) {
+    constexpr int l = decltype(lc)::value;
+    if constexpr (class_has_degree<N, C>(l)) phi[l] = expm1div(-Cl[l] / S * a.dt);
+  });
+  double q0 = 0.0;
+  if constexpr (MODE == kAdjoint && C == 0) {
+    // compute_adjoint_source! (x_ray_intensities.jl:155-170), eee m = 0 only
+    if (a.ibar != nullptr) {
+      const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
+      const double* __restrict__ ib = a.ibar + beam * L.n_lines;
+      for (int r = 0; r < L.n_lines; ++r) q0 -= sig[r] * a.W[r * V + p] * ib[r];
+    }
+  }
+  double* out = dst + L.cls_off[C];
+  static_for<0, NC>([&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    constexpr int l = Tab<N>::t.l[C][I];
+    double t;
+    if constexpr (MODE == kAdjoint) {
+      const double q = (C == 0 && I == 0) ? q0 : 0.0;
+      t = (q - du[I]) - Cl[l] * u[I];
+    } else {
+      t = (-du[I]) - Cl[l] * u[I];
+    }
+    t = t / S;
+    t = t * phi[l];
+    const double un = u[I] + a.dt * t;
+    out[I * V + p] = un;
+    if constexpr (MODE == kForward && PAR == 0 && C == 0 && I == 0) {
+      for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = a.W[r * V + p] * un;
+    }
+  });
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Maps the linear block id to (beam, tile) so that each XCD (blocks are
+// dealt round-robin over 8 XCDs) works on a contiguous range of tiles and the
+// stencil's neighbour rows stay in that XCD's L2.
+__device__ __forceinline__ void block_coords(int n_tiles, int n_beams, int& beam, int& tile) {
+  const int total = n_tiles * n_beams;
+  int logical = blockIdx.x;
+  if ((total & 7) == 0) logical = (blockIdx.x & 7) * (total >> 3) + (blockIdx.x >> 3);
+  beam = logical / n_tiles;
+  tile = logical - beam * n_tiles;
+}
+
+template <int N, int DIM, int PAR, int MODE>
+__global__ __launch_bounds__(kBlock) void half_step_kernel(HalfStepArgs a) {
+  const Layout& L = a.L;
+  int beam, tile;
+  block_coords(L.n_tiles, a.n_beams, beam, tile);
+  const int q = tile * kBlock + threadIdx.x;
+  const bool valid = q < L.n_cells;
+  int i = 0, j = 0, k = 0;
+  if (valid) {
+    i = q % L.Mx;
+    const int r = q / L.Mx;
+    j = r % L.My;
+    k = r / L.My;
+  }
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const long bo = (long)beam * L.beam_stride;
+  double contrib[kMaxLines];
+#pragma unroll
+  for (int r = 0; r < kMaxLines; ++r) contrib[r] = 0.0;
+  if (valid) {
+    static_for<0, 8>([&](auto Cc) {
+      constexpr int C = decltype(Cc)::value;
+      if constexpr (parity_of(C) == PAR && class_active(C, DIM) && ncls<N, C>() > 0) {
+        const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                                (DIM >= 3 && odd_in(C, 2) && k == 0));
+        if (interior)
+          update_class<N, DIM, C, MODE, PAR>(a, a.from + bo, a.to_src + bo, a.to_dst + bo, i, j, k, p, beam, contrib);
+      }
+    });
+  }
+  if constexpr (MODE == kForward && PAR == 0) {
+    __shared__ double red[kBlock / 64][kMaxLines];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) {
+      const double s = wave_sum(contrib[r]);
+      if (lane == 0) red[wave][r] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < L.n_lines && threadIdx.x < kMaxLines) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+      a.partial[beam * a.partial_stride + (long)tile * L.n_lines + threadIdx.x] = s;
+    }
+  }
+}
+
+// integrate_adjoint_densities! (pn_solver/pn_intensities.jl:129-187) for one
+// reverse step: du of every class without the exponential factor, then
+// rho_bar += lambda (s du + c u) dvol dE int_fac, plus the x-ray generation
+// pullback of integrate_step_and_add_pullback! at eee points.
+template <int N, int DIM, int C>
+__device__ __forceinline__ void dens_class(const DensArgs& a, const double* __restrict__ u_b,
+                                           const double* __restrict__ lam_b, int i, int j, int k, long p, int beam) {
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  const int ne = L.n_e;
+  const double* __restrict__ uc = u_b + L.cls_off[C];
+  double u[NC], du[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    u[m] = uc[m * V + p];
+    du[m] = 0.0;
+  }
+  const double gfac = a.rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, true>(L, u_b, a.coef, u, du, i, j, k, p, gfac, a.prof + beam * a.prof_stride, a.mms);
+  double S, Cl[N + 1];
+  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
+  const double* __restrict__ lc = lam_b + L.cls_off[C];
+  double lam[NC], t[NC];
+  static_for<0, NC>([&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    constexpr int l = Tab<N>::t.l[C][I];
+    lam[I] = lc[I * V + p];
+    t[I] = ((-du[I]) - Cl[l] * u[I]) / S;
+  });
+  double* __restrict__ rb = a.rho_bar + beam * a.rb_stride + a.rb_off[C];
+  const double* __restrict__ s = a.rec;
+  const double* __restrict__ cl = a.rec + ne;
+  const double w = a.L.dvol;
+  for (int e = 0; e < ne; ++e) {
+    double acc = rb[e * V + p];
+    const double se = s[e];
+    static_for<0, NC>([&](auto Ic) {
+      constexpr int I = decltype(Ic)::value;
+      constexpr int l = Tab<N>::t.l[C][I];
+      acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * a.int_fac;
+    });
+    rb[e * V + p] = acc;
+  }
+  if constexpr (C == 0) {
+    const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
+    const double* __restrict__ ib = a.ibar + beam * L.n_lines;
+    double* __restrict__ g = a.gen + beam * a.gen_stride;
+    for (int r = 0; r < L.n_lines; ++r) {
+      const double integral = a.dE * w * sig[r] * a.int_fac * ib[r];
+      g[r * V + p] += u[0] * integral;
+    }
+  }
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
+  const Layout& L = a.L;
+  int beam, tile;
+  block_coords(L.n_tiles, a.n_beams, beam, tile);
+  const int q = tile * kBlock + threadIdx.x;
+  if (q >= L.n_cells) return;
+  const int i = q % L.Mx;
+  const int r = q / L.Mx;
+  const int j = r % L.My;
+  const int k = r / L.My;
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const long bo = (long)beam * L.beam_stride;
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (class_active(C, DIM) && ncls<N, C>() > 0) {
+      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                              (DIM >= 3 && odd_in(C, 2) && k == 0));
+      if (interior) dens_class<N, DIM, C>(a, a.u + bo, a.lam + bo, i, j, k, p, beam);
+    }
+  });
+}
+
+template <int N>
+hipError_t half_step_order(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
+  const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
+  switch (dims * 4 + parity * 2 + mode) {
+#define PNX_CASE(D, P, M) \
+  case D * 4 + P * 2 + M: hipLaunchKernelGGL((half_step_kernel<N, D, P, M>), grid, block, 0, s, a); break;
+    PNX_CASE(1, 0, 0) PNX_CASE(1, 0, 1) PNX_CASE(1, 1, 0) PNX_CASE(1, 1, 1)
+    PNX_CASE(2, 0, 0) PNX_CASE(2, 0, 1) PNX_CASE(2, 1, 0) PNX_CASE(2, 1, 1)
+    PNX_CASE(3, 0, 0) PNX_CASE(3, 0, 1) PNX_CASE(3, 1, 0) PNX_CASE(3, 1, 1)
+#undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t dens_order(int dims, const DensArgs& a, hipStream_t s) {
+  const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((dens_kernel<N, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((dens_kernel<N, 2>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((dens_kernel<N, 3>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pnx

@@ -1,0 +1,12 @@
+// Instantiation of the PN-CSD kernels for PN order 3.
+#include "pnx_kernels_impl.hpp"
+
+namespace pnx {
+
+hipError_t half_step_n3(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
+  return half_step_order<3>(dims, parity, mode, a, s);
+}
+
+hipError_t dens_n3(int dims, const DensArgs& a, hipStream_t s) { return dens_order<3>(dims, a, s); }
+
+}  // namespace pnx
