@@ -1,19 +1,26 @@
 """Benchmark of the PN-CSD forward+adjoint hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serve|raster2d_grad] [--requests R]
 
-A "step" is one forward+adjoint solve of one batch of the workload (every beam
-of the batch is one solve).  The timed region covers exactly K steps with the
-inputs already resident in HBM, bracketed by device synchronisation.  One JSON
-line is printed with the throughput in solves/s, the roofline figure of the
-dominant kernel (the fused half-step stencil, measured with HIP events on the
-library's stream) and a CPU baseline (the C oracle, one core, on a bounded
-sample of the same workload).
+serve (default): the mixed serving stream -- R forward+gradient requests (PN
+order 3-9, 1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements) with
+Poisson arrivals, served by the continuous-batching server under an HBM
+budget.  A "step" is one pass of the whole stream; the timed region covers K
+passes with all request inputs prepared in host memory beforehand.  Reported
+beside it: p50/p99 latency (arrival to result), the plain one-request-at-a-time
+GPU loop on a sample of the same requests, and the C oracle on one core.
+
+raster2d_grad: 2D P7 128x96, 32 beams, forward+adjoint of the batch per step.
+
+Every mode also measures the roofline figure of the dominant kernel (the fused
+even-parity half step of the 2D P7 128x96 batch) with HIP events on the
+library's stream.  One JSON line is printed.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,77 +29,175 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "served fwd+adj solves/s, p50/p99 latency vs plain GPU loop @ k-ratio err<=1e-10"
 
 
-def build_workload(name):
+def _prep_one(spec):
+    from pnepma_amd import serving
+    return serving.prepare(spec)
+
+
+def prepare_stream(n, seed=3):
+    """Draw and prepare the request stream on the host (process pool, before the GPU is touched)."""
+    import multiprocessing as mp
+    from pnepma_amd import serving, physics
+    for e in physics.ELEMENTS.values():
+        physics.transport_coefficients(e, 1000.0, 9)
+    specs = serving.serving_requests(n, seed=seed)
+    workers = max(1, min(16, (os.cpu_count() or 1)))
+    with mp.get_context("fork").Pool(workers) as pool:
+        preps = pool.map(_prep_one, specs, chunksize=8)
+    return specs, preps
+
+
+def roofline_probe():
+    """Fused even-parity half step of the 2D P7 128x96 32-beam batch."""
     from pnepma_amd import workloads
-    if name == "raster2d_grad":
-        # 2D P7 128x96, Fe/Cr/Ni inclusions, 32 beams, 20 -> 1 keV; the stated 200 energy
-        # steps are run as 400 (CFL 0.45), checkpoints every 16 steps
-        p, mat, stds, det = workloads.raster2d(n_steps=400)
-        return p, mat, stds, det, 16
-    if name == "slab1d_grad":
-        p, mat, stds, det = workloads.slab1d(n_steps=300)
-        return p, mat, stds, det, 16
-    raise SystemExit(f"unknown workload {name}")
-
-
-def cpu_baseline(p, mat, det, si, ib, budget_s=20.0):
-    """Oracle forward+adjoint of beam 0 on a bounded number of energy steps."""
-    from oracle import oracle
-    import copy
-    import numpy as np
-    q = copy.copy(p)
-    q._cache = {}
-    q.beam_positions = [p.beam_positions[0]]
-    # probe cost with a short run, then size the sample to the budget
-    n_probe = max(4, p.n_steps // 50)
-    q.n_steps = n_probe
-    q.E_cutoff = p.E_initial - p.dE * n_probe
-    t = time.perf_counter()
-    oracle.forward_adjoint(q, si, ib[0])
-    per_step = (time.perf_counter() - t) / n_probe
-    n = int(max(n_probe, min(p.n_steps, budget_s / max(per_step, 1e-9))))
-    q = copy.copy(p)
-    q._cache = {}
-    q.beam_positions = [p.beam_positions[0]]
-    q.n_steps = n
-    q.E_cutoff = p.E_initial - p.dE * n
-    t = time.perf_counter()
-    oracle.forward_adjoint(q, si, ib[0])
-    el = time.perf_counter() - t
-    full_solve_s = el * p.n_steps / n
-    return {"value": 1.0 / full_solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle forward+adjoint of 1 beam, {n} of {p.n_steps} energy steps in {el:.1f} s, "
-                      f"scaled to the full step count"}
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="raster2d_grad")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
-    import torch
     from pnepma_amd.problem import solve_inputs
     from pnepma_amd.solver import DeviceProblem
+    p, mat, stds, det = workloads.raster2d(n_steps=400)
+    dp = DeviceProblem(p, solve_inputs(p, mat, det), adjoint=False)
+    ms, by = dp.half_step_bench(0, 50)
+    achieved = by / (ms * 1e-3) / 1e9
+    dp.close()
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "half_step_kernel<7,2,0,fwd> (even parity, 2D P7 128x96, 32 beams)",
+            "kernel_ms": ms, "algorithmic_bytes": by}
 
+
+def serve_run(preps, rate, budget, max_active):
+    """Submit every prepared request with Poisson arrivals of the given rate; returns
+    (wall seconds from first arrival to last completion, latencies in seconds, stats)."""
+    import numpy as np
+    from pnepma_amd import serving
+    rng = np.random.default_rng(7)
+    gaps = rng.exponential(1.0 / rate, len(preps)) if rate > 0 else np.zeros(len(preps))
+    arr = np.cumsum(gaps)
+    arr -= arr[0]
+    srv = serving.Server(budget, max_active=max_active)
+    t0 = srv.clock() + 0.05
+    tickets = [srv.submit(pr, arrival_s=t0 + a) for pr, a in zip(preps, arr)]
+    srv.drain()
+    tm = [srv.wait(t) for t in tickets]
+    stats = srv.stats()
+    srv.close()
+    lat = np.array([x.done_s - x.arrival_s for x in tm])
+    wall = max(x.done_s for x in tm) - t0
+    bad = sum(1 for x in tm if x.status != 0)
+    if bad:
+        raise RuntimeError(f"{bad} served requests failed")
+    return wall, lat, stats
+
+
+def sequential_run(specs, preps, max_seconds=20.0, max_requests=64):
+    """The plain loop: one request at a time with the same kernels, no batching, no graph."""
+    import numpy as np
+    from pnepma_amd.solver import DeviceProblem
+    times, outs = [], []
+    t_all = time.perf_counter()
+    for spec, pr in zip(specs, preps):
+        K = max(1, math.ceil(math.sqrt(spec.problem.n_steps)))
+        dp = DeviceProblem(spec.problem, pr.si, ckpt_every=K)
+        t = time.perf_counter()
+        I, _, _ = dp.forward_adjoint(spec.I_bar[None, :])
+        times.append(time.perf_counter() - t)
+        outs.append(I[0])
+        dp.close()
+        if len(times) >= max_requests or time.perf_counter() - t_all > max_seconds:
+            break
+    return np.array(times), outs
+
+
+def cpu_sample(specs, preps, budget_s=20.0):
+    """The C oracle (single core) on the first requests of the stream until the budget is spent."""
+    from oracle import oracle
+    t_all = time.perf_counter()
+    n, el = 0, 0.0
+    for spec, pr in zip(specs, preps):
+        t = time.perf_counter()
+        oracle.forward_adjoint(spec.problem, pr.si, spec.I_bar)
+        el += time.perf_counter() - t
+        n += 1
+        if time.perf_counter() - t_all > budget_s:
+            break
+    return n, el
+
+
+def main_serve(args):
+    import numpy as np
+    specs, preps = prepare_stream(args.requests)
+    import torch
     assert torch.cuda.is_available()
     torch.cuda.init()
-    p, mat, stds, det, ckpt = build_workload(args.workload)
+    torch.zeros(1, device="cuda:0")
+    from pnepma_amd import serving
+    free, total = torch.cuda.mem_get_info()
+    budget = int(min(free * 0.7, 200e9))
+    max_active = 4096
+    # capacity probe on the head of the stream, all arriving at once
+    head = preps[:min(len(preps), 256)]
+    w0, _, _ = serve_run(head, 0.0, budget, max_active)
+    capacity = len(head) / w0
+    rate = 1.2 * capacity
+    for _ in range(max(0, args.warmup - 1)):
+        serve_run(preps[:64], rate, budget, max_active)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    walls, lats, stats = [], [], None
+    for _ in range(args.steps):
+        w, lat, stats = serve_run(preps, rate, budget, max_active)
+        walls.append(w)
+        lats.append(lat)
+    torch.cuda.synchronize()
+    total_wall = time.perf_counter() - t0
+    lat = np.concatenate(lats)
+    served = len(preps) * args.steps / sum(walls)
+    # results of the served pass vs the same requests alone (plain loop)
+    seq_t, seq_I = sequential_run(specs, preps)
+    seq_rate = len(seq_t) / seq_t.sum()
+    # the plain-loop sample is the head of the stream: compare with the served results
+    alone_err = max(float(np.max(np.abs(a - pr.I_out)) / np.max(np.abs(a))) for a, pr in zip(seq_I, preps))
+    n_cpu, el_cpu = cpu_sample(specs, preps) if not args.no_cpu else (0, 0.0)
+    roof = roofline_probe()
+    out = {
+        "metric": METRIC, "value": served, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total_wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving stream of {len(preps)} forward+gradient requests: PN 3/5/7/9, "
+                               "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, CFL-0.5 step counts, "
+                               "Poisson arrivals at 1.2x measured capacity",
+                   "requests": len(preps), "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
+                   "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "sample": f"first {len(seq_t)} requests of "
+                           "the stream, one at a time, same kernels, no batching, no graph",
+                           "mean_ms": float(seq_t.mean() * 1e3)},
+        "speedup_vs_plain_gpu_loop": served / seq_rate,
+        "served_vs_alone_max_rel_err": alone_err,
+        "server_stats": stats,
+        "roofline": roof,
+    }
+    if n_cpu:
+        out["cpu_baseline"] = {"value": n_cpu / el_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"C oracle forward+adjoint of the first {n_cpu} requests of the stream "
+                                         f"({el_cpu:.1f} s on one core)"}
+        out["speedup_vs_cpu"] = served / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+def main_raster(args):
+    import numpy as np
+    import torch
+    from pnepma_amd import workloads
+    from pnepma_amd.problem import solve_inputs
+    from pnepma_amd.solver import DeviceProblem
+    assert torch.cuda.is_available()
+    torch.cuda.init()
+    p, mat, stds, det = workloads.raster2d(n_steps=400)
     si = solve_inputs(p, mat, det)
-    dp = DeviceProblem(p, si, ckpt_every=ckpt)
-    rng = np.random.default_rng(2)
-    ib = rng.uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
-    # parity spot check of beam 0 against the oracle is done on the CPU sample below
+    dp = DeviceProblem(p, si, ckpt_every=16)
+    ib = np.random.default_rng(2).uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
     I, rb, gen = dp.forward_adjoint(ib)
     for _ in range(max(0, args.warmup - 1)):
         dp.forward_adjoint_timed(1)
@@ -101,43 +206,42 @@ def main():
     dp.forward_adjoint_timed(args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    solves_per_s = p.n_beams * args.steps / wall
-
-    # roofline of the dominant kernel: fused even-parity half step (forward)
-    ms_k, bytes_k = dp.half_step_bench(0, 50)
-    achieved = bytes_k / (ms_k * 1e-3) / 1e9
     fwd_ms = dp.forward_timed(3, True)
-
+    dp.close()
     out = {
-        "metric": "served fwd+adj solves/s, p50/p99 latency vs plain GPU loop @ k-ratio err<=1e-10",
-        "value": solves_per_s,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: 2D P7 128x96, 32 beams, Fe/Cr/Ni, 400 energy steps, "
-                               f"checkpoints every {ckpt}, batch of 32 forward+adjoint solves per step"
-                   if args.workload == "raster2d_grad" else args.workload,
-                   "n_beams": p.n_beams, "n_steps": p.n_steps, "N": p.N, "grid": list(p.gspec.M)},
+        "metric": METRIC, "value": p.n_beams * args.steps / wall, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "raster2d_grad: 2D P7 128x96, 32 beams, Fe/Cr/Ni, 400 energy steps (CFL 0.45), "
+                               "checkpoints every 16, one forward+adjoint of the 32-beam batch per step",
+                   "n_beams": p.n_beams, "n_steps": p.n_steps},
         "forward_ms_per_batch": fwd_ms,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "half_step_kernel even parity forward", "kernel_ms": ms_k,
-                     "algorithmic_bytes": bytes_k},
+        "roofline": roofline_probe(),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, mat, det, si, ib)
         from oracle import oracle
         I_ref = oracle.forward(p, si, beam=0)
         out["parity_beam0_intensity_rel_err"] = float(np.max(np.abs(I[0] - I_ref)) / np.max(np.abs(I_ref)))
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad"])
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(how the path would shard over GPUs is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "serve":
+        main_serve(args)
+    else:
+        main_raster(args)
 
 
 if __name__ == "__main__":

@@ -101,6 +101,17 @@ int pnx_forward(pnx_problem* p, double* I_out);
  *                          (x_ray_intensities.jl:193-223). May be NULL. */
 int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out);
 
+/* Exact gradient of sum_{b,r} I_bar[b][r] I[b][r] of the discrete forward
+ * march (the transpose of every half step, including the exponential factor
+ * and the boundary ghosts), with the same checkpointing.  Outputs as
+ * pnx_forward_adjoint; gen_out here is d/dW (sigma at the forward energies).
+ * The reference's adjoint (pnx_forward_adjoint) is a discretised continuous
+ * adjoint and agrees with finite differences only to discretisation error;
+ * this variant agrees with central differences of pnx_forward to their
+ * truncation error.  It has no counterpart in the reference. */
+int pnx_forward_adjoint_exact(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8],
+                              double* gen_out);
+
 /* One energy step (step_pde!) on a host state for beam `beam`: u[c] is the
  * reference's [point][moment] array of class c (ghost cells ignored on input,
  * undefined on output); classes the grid never updates may be NULL.
@@ -120,6 +131,55 @@ int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out);
  * back-to-back launches on the problem's stream, and the algorithmic bytes
  * of one such launch. */
 int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, double* bytes_out);
+
+/* ------------------------------------------------------------------ serving
+ * A continuous-batching server of forward+adjoint requests with mixed PN
+ * order, grid and energy range.  Requests are admitted, in arrival order, as
+ * soon as their arrival time has passed and their device memory fits in the
+ * HBM budget; every scheduler tick advances each admitted request by one
+ * energy step (forward, recomputation or reverse) in persistent kernels that
+ * batch all requests of one PN order and dimension.  The reference has no
+ * counterpart (it solves one problem at a time); each request runs the same
+ * march as pnx_forward_adjoint with one beam. */
+typedef struct pnx_server pnx_server;
+
+typedef struct {
+  pnx_desc desc;                 /* n_beams must be 1; ckpt_every 0 = about sqrt(n_steps) */
+  const double* fwd;             /* records as pnx_problem_set_records */
+  const double* adj;
+  const double* dens;
+  const double* rho[8];          /* as pnx_problem_set_material */
+  const double* W;
+  const double* mms[8];          /* as pnx_problem_set_beams, one beam */
+  const double* ppos[8];
+  const double* I_bar;           /* n_lines pullback seed */
+  double arrival_s;              /* admission not before this server-clock time */
+  double* I_out;                 /* n_lines, written at completion */
+  double* rho_bar[8];            /* optional [point][element] per class, written at completion */
+  double* gen_out;               /* optional [eee point][line], written at completion */
+} pnx_request;
+
+typedef struct {
+  double arrival_s;   /* server clock, seconds */
+  double admit_s;
+  double done_s;
+  int32_t status;     /* PNX_OK or an error code */
+  int32_t ticks;      /* scheduler ticks the request took part in */
+} pnx_timing;
+
+/* Inputs are copied at submit; output pointers must stay valid until done. */
+int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_active, pnx_server** out);
+int pnx_server_destroy(pnx_server* s);
+double pnx_server_clock(const pnx_server* s);
+int pnx_server_submit(pnx_server* s, const pnx_request* req, int64_t* ticket);
+int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t);
+int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t);
+/* Block until every submitted request has completed. */
+int pnx_server_drain(pnx_server* s);
+/* ticks, completed requests, peak concurrently admitted requests, peak device bytes in use */
+int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active, int64_t* peak_bytes);
+/* Device bytes a request would hold while admitted. */
+int64_t pnx_request_device_bytes(const pnx_desc* desc);
 
 /* Matrices the kernels use, for inspection: the coefficient buffer of order N
  * (adjoint = 0 forward, 1 adjoint): all advection nonzeros, then dense

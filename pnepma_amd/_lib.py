@@ -21,6 +21,17 @@ class Desc(C.Structure):
                 ("E_initial", C.c_double), ("E_cutoff", C.c_double), ("ckpt_every", C.c_int32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("desc", Desc), ("fwd", _dp), ("adj", _dp), ("dens", _dp), ("rho", _dp * 8), ("W", _dp),
+                ("mms", _dp * 8), ("ppos", _dp * 8), ("I_bar", _dp), ("arrival_s", C.c_double), ("I_out", _dp),
+                ("rho_bar", _dp * 8), ("gen_out", _dp)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
+                ("ticks", C.c_int32)]
+
+
 # name -> (restype, argtypes); every symbol include/pnx.h declares
 SIGNATURES = {
     "pnx_last_error": (C.c_char_p, []),
@@ -36,10 +47,21 @@ SIGNATURES = {
     "pnx_problem_set_beams": (C.c_int, [C.c_void_p, _dp * 8, C.POINTER(_dp)]),
     "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
     "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, _dp * 8, _dp]),
+    "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, _dp * 8, _dp]),
     "pnx_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp * 8]),
     "pnx_forward_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pnx_forward_adjoint_timed": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "pnx_half_step_bench": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pnx_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "pnx_server_destroy": (C.c_int, [C.c_void_p]),
+    "pnx_server_clock": (C.c_double, [C.c_void_p]),
+    "pnx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "pnx_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(Timing)]),
+    "pnx_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Timing)]),
+    "pnx_server_drain": (C.c_int, [C.c_void_p]),
+    "pnx_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
+    "pnx_request_device_bytes": (C.c_int64, [C.POINTER(Desc)]),
     "pnx_coefficients": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int]),
     "pnx_advection_block": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp]),
     "pnx_boundary_block": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp]),

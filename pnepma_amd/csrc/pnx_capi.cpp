@@ -14,33 +14,32 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pnx.h"
-#include "pnx_kernels.hpp"
-#include "pnx_tables.hpp"
+#include "pnx_host.hpp"
 
 namespace pnx {
-std::vector<double> build_coefficients(int N, bool adjoint);
 double advection_coefficient(int l, int k, int lf, int kf, int d);
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
 }  // namespace pnx
 
 using namespace pnx;
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 #define HIP_TRY(expr)                                                                               \
   do {                                                                                              \
     hipError_t e_ = (expr);                                                                         \
     if (e_ != hipSuccess) return fail(PNX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
-
-bool order_supported(int N) { return N == 1 || N == 3 || N == 5 || N == 7 || N == 9; }
 
 struct DevBuf {
   double* p = nullptr;
@@ -66,33 +65,17 @@ struct DevBuf {
 
 }  // namespace
 
-struct pnx_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-};
-
 struct pnx_problem {
   pnx_context* ctx = nullptr;
   pnx_desc d{};
-  Layout L{};
-  Tables T{};
-  int rec_len = 0;
-  int sigma_off = 0;
-  double dE = 0.0;
-  int nck = 0;          // checkpoints
-  int K = 0;            // checkpoint interval
-  long rho_off[8]{};
-  long rho_total = 0;
-  long rb_off[8]{};
+  Geometry g{};
+  Layout& L = g.L;
+  Tables& T = g.T;
   long rb_stride = 0;   // per beam rho_bar doubles
-  long prof_stride = 0;
-  long partial_step = 0, partial_beam = 0;
-  int class_n[8][3]{};  // reference class sizes
   bool have_rec = false, have_adj = false, have_mat = false, have_beams = false;
   DevBuf state, rho, W, coef_f, coef_a, rec_f, rec_a, rec_d, prof, mms, partial, I_dev;
-  DevBuf lam, ckpt, seg, rho_bar_b, gen_b, ibar, sum_buf;
+  DevBuf lam, ckpt, seg, rho_bar_b, gen_b, ibar, sum_buf, w1, w2;
   hipGraphExec_t fwd_graph = nullptr;
-  std::vector<double> host_tmp;
 };
 
 static void destroy_graph(pnx_problem* p) {
@@ -170,85 +153,24 @@ int pnx_boundary_block(int N, int c, int d, int adjoint, double* out) {
 int pnx_problem_create(pnx_context* ctx, const pnx_desc* desc, pnx_problem** out) {
   if (!ctx || !desc || !out) return fail(PNX_ERR_ARG, "null argument");
   const pnx_desc& d = *desc;
-  if (!order_supported(d.N)) return fail(PNX_ERR_UNSUPPORTED, "PN order must be one of 1, 3, 5, 7, 9");
-  if (d.n_elements < 1 || d.n_elements > 8) return fail(PNX_ERR_ARG, "n_elements must be 1..8");
-  if (d.n_lines < 1 || d.n_lines > 8) return fail(PNX_ERR_ARG, "n_lines must be 1..8");
-  if (d.n_steps < 1 || d.n_beams < 1) return fail(PNX_ERR_ARG, "n_steps and n_beams must be positive");
-  if (!(d.E_initial > d.E_cutoff)) return fail(PNX_ERR_ARG, "E_initial must exceed E_cutoff");
-  int dims = 0;
-  for (int a = 0; a < 3; ++a) {
-    if (d.M[a] < 1) return fail(PNX_ERR_ARG, "grid sizes must be >= 1");
-    if (d.M[a] == 1) continue;
-    if (d.M[a] < 2 || a != dims) return fail(PNX_ERR_ARG, "non-singleton axes must come first (x, then y, then z)");
-    if (!(d.hi[a] > d.lo[a])) return fail(PNX_ERR_ARG, "axis extent must be positive");
-    ++dims;
-  }
-  if (dims == 0) return fail(PNX_ERR_ARG, "at least the x axis must be resolved");
   HIP_TRY(hipSetDevice(ctx->device));
   auto* p = new pnx_problem;
   p->ctx = ctx;
   p->d = d;
-  p->T = make_tables(d.N);
-  Layout& L = p->L;
-  L.Mx = d.M[0];
-  L.My = d.M[1];
-  L.Mz = d.M[2];
-  L.dims = dims;
-  L.N = d.N;
-  L.n_e = d.n_elements;
-  L.n_lines = d.n_lines;
-  // x pitch padded to 16 doubles (128 B cache lines) once rows are long enough
-  const int xneed = L.Mx + 1;
-  L.X = xneed >= 64 ? (xneed + 15) / 16 * 16 : xneed;
-  L.Y = L.My == 1 ? 1 : L.My + 1;
-  L.Z = L.Mz == 1 ? 1 : L.Mz + 1;
-  L.V = (long)L.X * L.Y * L.Z;
-  long off = 0;
-  for (int c = 0; c < 8; ++c) {
-    if (class_active(c, dims) && p->T.n[c] > 0) {
-      L.cls_off[c] = off;
-      off += p->T.n[c] * L.V;
-    } else {
-      L.cls_off[c] = 0;  // never touched by the kernels
-    }
+  std::string err;
+  if (int e = make_geometry(d, p->g, err)) {
+    delete p;
+    return fail(e, err);
   }
-  L.beam_stride = off;
-  double h[3];
-  for (int a = 0; a < 3; ++a) {
-    h[a] = d.M[a] == 1 ? 1.0 : (d.hi[a] - d.lo[a]) / (d.M[a] - 1);
-    L.inv_h[a] = 1.0 / h[a];
-    for (int c = 0; c < 8; ++c) p->class_n[c][a] = d.M[a] == 1 ? 1 : (odd_in(c, a) ? d.M[a] + 1 : d.M[a]);
-  }
-  L.dvol = h[0] * h[1] * h[2];
-  L.n_cells = L.Mx * L.My * L.Mz;
-  L.n_tiles = (L.n_cells + kBlock - 1) / kBlock;
-  p->rec_len = d.n_elements + d.n_elements * (d.N + 1) + 1 + d.n_lines;
-  L.rec_len = p->rec_len;
-  p->sigma_off = d.n_elements + d.n_elements * (d.N + 1) + 1;
-  p->dE = (d.E_initial - d.E_cutoff) / d.n_steps;
-  p->K = d.ckpt_every > 0 ? std::min(d.ckpt_every, d.n_steps) : d.n_steps;
-  p->nck = (d.n_steps + p->K - 1) / p->K;
-  long ro = 0, rbo = 0;
-  for (int c = 0; c < 8; ++c) {
-    const bool act = class_active(c, dims) && p->T.n[c] > 0;
-    p->rho_off[c] = act ? ro : 0;
-    p->rb_off[c] = act ? rbo : 0;
-    if (act) {
-      ro += (long)d.n_elements * L.V;
-      rbo += (long)d.n_elements * L.V;
-    }
-  }
-  p->rho_total = ro;
-  p->rb_stride = rbo;
-  p->prof_stride = 8L * L.Y * L.Z;
-  p->partial_step = (long)L.n_tiles * d.n_lines;
-  p->partial_beam = (long)(d.n_steps + 1) * p->partial_step;
+  const Layout& L = p->L;
+  const long ro = p->g.rho_total;
+  p->rb_stride = ro;
   int rc = 0;
   if ((rc = p->state.alloc((size_t)L.beam_stride * d.n_beams)) || (rc = p->rho.alloc(ro)) ||
-      (rc = p->W.alloc((size_t)d.n_lines * L.V)) || (rc = p->prof.alloc((size_t)p->prof_stride * d.n_beams)) ||
-      (rc = p->mms.alloc(8 * 16)) || (rc = p->partial.alloc((size_t)p->partial_beam * d.n_beams)) ||
+      (rc = p->W.alloc((size_t)d.n_lines * L.V)) || (rc = p->prof.alloc((size_t)p->g.prof_stride * d.n_beams)) ||
+      (rc = p->mms.alloc(8 * 16)) || (rc = p->partial.alloc((size_t)p->g.partial_beam * d.n_beams)) ||
       (rc = p->I_dev.alloc((size_t)d.n_beams * d.n_lines)) ||
-      (rc = p->rec_f.alloc((size_t)(d.n_steps + 1) * p->rec_len))) {
+      (rc = p->rec_f.alloc((size_t)(d.n_steps + 1) * p->g.rec_len))) {
     delete p;
     return rc;
   }
@@ -286,7 +208,7 @@ int64_t pnx_problem_device_bytes(const pnx_problem* p) {
   size_t n = 0;
   for (const DevBuf* b : {&p->state, &p->rho, &p->W, &p->coef_f, &p->coef_a, &p->rec_f, &p->rec_a, &p->rec_d,
                           &p->prof, &p->mms, &p->partial, &p->I_dev, &p->lam, &p->ckpt, &p->seg, &p->rho_bar_b,
-                          &p->gen_b, &p->ibar, &p->sum_buf})
+                          &p->gen_b, &p->ibar, &p->sum_buf, &p->w1, &p->w2})
     n += b->n;
   return (int64_t)(n * sizeof(double));
 }
@@ -295,7 +217,7 @@ int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj
   if (!p || !fwd) return fail(PNX_ERR_ARG, "null argument");
   if ((adj == nullptr) != (dens == nullptr)) return fail(PNX_ERR_ARG, "adj and dens must be given together");
   HIP_TRY(hipSetDevice(p->ctx->device));
-  const size_t n = (size_t)(p->d.n_steps + 1) * p->rec_len;
+  const size_t n = (size_t)(p->d.n_steps + 1) * p->g.rec_len;
   hipStream_t s = p->ctx->stream;
   HIP_TRY(hipMemcpyAsync(p->rec_f.p, fwd, n * sizeof(double), hipMemcpyHostToDevice, s));
   if (adj) {
@@ -313,30 +235,15 @@ int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj
 int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const double* W) {
   if (!p || !rho || !W) return fail(PNX_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(p->ctx->device));
-  const Layout& L = p->L;
-  const int ne = p->d.n_elements;
-  std::vector<double> h((size_t)p->rho_total, 0.0);
+  const Geometry& g = p->g;
+  std::vector<double> h((size_t)g.rho_total, 0.0);
   for (int c = 0; c < 8; ++c) {
-    if (!(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    if (!g.active(c)) continue;
     if (!rho[c]) return fail(PNX_ERR_ARG, "densities missing for an updated class");
-    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2];
-    for (int k = 0; k < nz; ++k)
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const long q = i + (long)nx * (j + (long)ny * k);
-          const long pd = i + (long)L.X * (j + (long)L.Y * k);
-          for (int e = 0; e < ne; ++e) h[p->rho_off[c] + e * L.V + pd] = rho[c][q * ne + e];
-        }
+    pack_class(g, c, rho[c], p->d.n_elements, h.data() + g.rho_off[c]);
   }
-  std::vector<double> w((size_t)p->d.n_lines * L.V, 0.0);
-  const int nl = p->d.n_lines;
-  for (int k = 0; k < L.Mz; ++k)
-    for (int j = 0; j < L.My; ++j)
-      for (int i = 0; i < L.Mx; ++i) {
-        const long q = i + (long)L.Mx * (j + (long)L.My * k);
-        const long pd = i + (long)L.X * (j + (long)L.Y * k);
-        for (int r = 0; r < nl; ++r) w[r * L.V + pd] = W[q * nl + r];
-      }
+  std::vector<double> w((size_t)p->d.n_lines * g.L.V, 0.0);
+  pack_lines(g, W, w.data());
   hipStream_t s = p->ctx->stream;
   HIP_TRY(hipMemcpyAsync(p->rho.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->W.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -348,20 +255,16 @@ int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const d
 int pnx_problem_set_beams(pnx_problem* p, const double* const mms[8], const double* const* ppos) {
   if (!p || !mms || !ppos) return fail(PNX_ERR_ARG, "null argument");
   HIP_TRY(hipSetDevice(p->ctx->device));
-  const Layout& L = p->L;
-  std::vector<double> m(8 * 16, 0.0), pr((size_t)p->prof_stride * p->d.n_beams, 0.0);
+  const Geometry& g = p->g;
+  std::vector<double> m(8 * 16, 0.0), pr((size_t)g.prof_stride * p->d.n_beams, 0.0);
   for (int c = 0; c < 8; ++c) {
-    if (!odd_in(c, 0) || !(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    if (!odd_in(c, 0) || !g.active(c)) continue;
     if (!mms[c]) return fail(PNX_ERR_ARG, "beam moments missing for an x-odd class");
-    for (int i = 0; i < p->T.n[c]; ++i) m[c * 16 + i] = mms[c][i];
-    const int ny = p->class_n[c][1], nz = p->class_n[c][2];
-    for (int b = 0; b < p->d.n_beams; ++b) {
-      const double* src = ppos[b * 8 + c];
-      if (!src) return fail(PNX_ERR_ARG, "beam profile missing for an x-odd class");
-      for (int k = 0; k < nz; ++k)
-        for (int j = 0; j < ny; ++j) pr[b * p->prof_stride + c * (long)L.Y * L.Z + j + (long)L.Y * k] = src[j + ny * k];
-    }
+    for (int i = 0; i < g.T.n[c]; ++i) m[c * 16 + i] = mms[c][i];
+    for (int b = 0; b < p->d.n_beams; ++b)
+      if (!ppos[b * 8 + c]) return fail(PNX_ERR_ARG, "beam profile missing for an x-odd class");
   }
+  for (int b = 0; b < p->d.n_beams; ++b) pack_profiles(g, ppos + b * 8, pr.data() + b * g.prof_stride);
   hipStream_t s = p->ctx->stream;
   HIP_TRY(hipMemcpyAsync(p->mms.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->prof.p, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -382,16 +285,16 @@ HalfStepArgs half_args(pnx_problem* p, int mode, const double* rec, double dt, c
   a.to_src = src;
   a.to_dst = dst;
   a.rho = p->rho.p;
-  for (int c = 0; c < 8; ++c) a.rho_off[c] = p->rho_off[c];
+  for (int c = 0; c < 8; ++c) a.rho_off[c] = p->g.rho_off[c];
   a.coef = mode == kForward ? p->coef_f.p : p->coef_a.p;
   a.rec = rec;
   a.prof = p->prof.p;
-  a.prof_stride = p->prof_stride;
+  a.prof_stride = p->g.prof_stride;
   a.mms = p->mms.p;
   a.W = p->W.p;
   a.ibar = mode == kAdjoint ? p->ibar.p : nullptr;
   a.partial = partial;
-  a.partial_stride = p->partial_beam;
+  a.partial_stride = p->g.partial_beam;
   a.dt = dt;
   a.n_beams = p->d.n_beams;
   return a;
@@ -401,22 +304,22 @@ HalfStepArgs half_args(pnx_problem* p, int mode, const double* rec, double dt, c
 // src -> dst (dst may equal src); partial receives the intensity partials of this step.
 int forward_step(pnx_problem* p, int k, const double* src, double* dst, double* partial) {
   hipStream_t s = p->ctx->stream;
-  const double* rec = p->rec_f.p + (long)k * p->rec_len;
+  const double* rec = p->rec_f.p + (long)k * p->g.rec_len;
   const int N = p->d.N, dims = p->L.dims;
-  const double h = p->dE / 2;
+  const double h = p->g.dE / 2;
   HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, src, src, dst, nullptr), s));
-  HIP_TRY(launch_half_step(N, dims, 0, kForward, half_args(p, kForward, rec, p->dE, dst, src, dst, partial), s));
+  HIP_TRY(launch_half_step(N, dims, 0, kForward, half_args(p, kForward, rec, p->g.dE, dst, src, dst, partial), s));
   HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, dst, dst, dst, nullptr), s));
   return PNX_OK;
 }
 
 int adjoint_step(pnx_problem* p, int i, double* lam) {
   hipStream_t s = p->ctx->stream;
-  const double* rec = p->rec_a.p + (long)i * p->rec_len;
+  const double* rec = p->rec_a.p + (long)i * p->g.rec_len;
   const int N = p->d.N, dims = p->L.dims;
-  const double h = p->dE / 2;
+  const double h = p->g.dE / 2;
   HIP_TRY(launch_half_step(N, dims, 1, kAdjoint, half_args(p, kAdjoint, rec, h, lam, lam, lam, nullptr), s));
-  HIP_TRY(launch_half_step(N, dims, 0, kAdjoint, half_args(p, kAdjoint, rec, p->dE, lam, lam, lam, nullptr), s));
+  HIP_TRY(launch_half_step(N, dims, 0, kAdjoint, half_args(p, kAdjoint, rec, p->g.dE, lam, lam, lam, nullptr), s));
   HIP_TRY(launch_half_step(N, dims, 1, kAdjoint, half_args(p, kAdjoint, rec, h, lam, lam, lam, nullptr), s));
   return PNX_OK;
 }
@@ -425,11 +328,11 @@ int enqueue_forward(pnx_problem* p) {
   hipStream_t s = p->ctx->stream;
   HIP_TRY(launch_fill(p->state.p, (long)p->L.beam_stride * p->d.n_beams, 0.0, s));
   for (int k = 1; k <= p->d.n_steps; ++k) {
-    int rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->partial_step);
+    int rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step);
     if (rc) return rc;
   }
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->partial_step, p->partial_beam, p->L.n_tiles, p->rec_f.p, p->rec_len,
-                                  p->sigma_off, p->d.n_steps, p->d.n_lines, p->dE, p->L.dvol, p->d.n_beams,
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
+                                  p->g.sigma_off, p->d.n_steps, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams,
                                   p->I_dev.p, s));
   return PNX_OK;
 }
@@ -464,15 +367,17 @@ int check_ready(pnx_problem* p, bool adjoint) {
   return PNX_OK;
 }
 
-int alloc_adjoint(pnx_problem* p) {
+int alloc_adjoint(pnx_problem* p, bool exact = false) {
   const size_t st = (size_t)p->L.beam_stride * p->d.n_beams;
   int rc;
-  if ((rc = p->lam.alloc(st)) || (rc = p->ckpt.alloc(st * p->nck)) || (rc = p->seg.alloc(st * p->K)) ||
+  if ((rc = p->lam.alloc(st)) || (rc = p->ckpt.alloc(st * p->g.nck)) ||
+      (rc = p->seg.alloc(st * p->g.K * (exact ? 2 : 1))) ||
       (rc = p->rho_bar_b.alloc((size_t)p->rb_stride * p->d.n_beams)) ||
       (rc = p->gen_b.alloc((size_t)p->d.n_lines * p->L.V * p->d.n_beams)) ||
       (rc = p->ibar.alloc((size_t)p->d.n_beams * p->d.n_lines)) ||
       (rc = p->sum_buf.alloc(std::max((size_t)p->rb_stride, (size_t)p->d.n_lines * p->L.V))))
     return rc;
+  if (exact && ((rc = p->w1.alloc(st)) || (rc = p->w2.alloc(st)))) return rc;
   return PNX_OK;
 }
 
@@ -483,13 +388,13 @@ DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
   a.lam = p->lam.p;
   a.rho = p->rho.p;
   for (int c = 0; c < 8; ++c) {
-    a.rho_off[c] = p->rho_off[c];
-    a.rb_off[c] = p->rb_off[c];
+    a.rho_off[c] = p->g.rho_off[c];
+    a.rb_off[c] = p->g.rho_off[c];
   }
   a.coef = p->coef_f.p;
-  a.rec = p->rec_d.p + (long)i * p->rec_len;
+  a.rec = p->rec_d.p + (long)i * p->g.rec_len;
   a.prof = p->prof.p;
-  a.prof_stride = p->prof_stride;
+  a.prof_stride = p->g.prof_stride;
   a.mms = p->mms.p;
   a.W = p->W.p;
   a.ibar = p->ibar.p;
@@ -497,7 +402,7 @@ DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
   a.rb_stride = p->rb_stride;
   a.gen = p->gen_b.p;
   a.gen_stride = (long)p->d.n_lines * p->L.V;
-  a.dE = p->dE;
+  a.dE = p->g.dE;
   a.int_fac = int_fac;
   a.n_beams = p->d.n_beams;
   return a;
@@ -508,21 +413,21 @@ DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
 int enqueue_forward_adjoint(pnx_problem* p) {
   hipStream_t s = p->ctx->stream;
   const long st = p->L.beam_stride * p->d.n_beams;
-  const int n = p->d.n_steps, K = p->K, N = p->d.N, dims = p->L.dims;
+  const int n = p->d.n_steps, K = p->g.K, N = p->d.N, dims = p->L.dims;
   int rc;
   HIP_TRY(launch_fill(p->state.p, st, 0.0, s));
   for (int k = 1; k <= n; ++k) {
     if ((k - 1) % K == 0)
       HIP_TRY(hipMemcpyAsync(p->ckpt.p + (long)((k - 1) / K) * st, p->state.p, st * sizeof(double),
                              hipMemcpyDeviceToDevice, s));
-    if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->partial_step))) return rc;
+    if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step))) return rc;
   }
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->partial_step, p->partial_beam, p->L.n_tiles, p->rec_f.p, p->rec_len,
-                                  p->sigma_off, n, p->d.n_lines, p->dE, p->L.dvol, p->d.n_beams, p->I_dev.p, s));
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
+                                  p->g.sigma_off, n, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams, p->I_dev.p, s));
   HIP_TRY(launch_fill(p->lam.p, st, 0.0, s));
   HIP_TRY(launch_fill(p->rho_bar_b.p, (long)p->rho_bar_b.n, 0.0, s));
   HIP_TRY(launch_fill(p->gen_b.p, (long)p->gen_b.n, 0.0, s));
-  for (int sgi = p->nck - 1; sgi >= 0; --sgi) {
+  for (int sgi = p->g.nck - 1; sgi >= 0; --sgi) {
     const int k0 = sgi * K, k1 = std::min(n, k0 + K);
     // recompute u_{k0+1} .. u_{k1} into the segment buffer
     const double* src = p->ckpt.p + (long)sgi * st;
@@ -542,6 +447,88 @@ int enqueue_forward_adjoint(pnx_problem* p) {
   return PNX_OK;
 }
 
+DiscArgs disc_args(pnx_problem* p, int k, const double* s_from, const double* s_own, const double* w_in,
+                   double* w_out, double tau, const double* u_direct) {
+  DiscArgs a{};
+  a.L = p->L;
+  a.s_from = s_from;
+  a.s_own = s_own;
+  a.lam = p->lam.p;
+  a.w_in = w_in;
+  a.w_out = w_out;
+  a.rho = p->rho.p;
+  for (int c = 0; c < 8; ++c) a.rho_off[c] = p->g.rho_off[c];
+  a.coef = p->coef_f.p;
+  a.rec = p->rec_f.p + (long)k * p->g.rec_len;
+  a.prof = p->prof.p;
+  a.prof_stride = p->g.prof_stride;
+  a.mms = p->mms.p;
+  a.W = p->W.p;
+  a.ibar = p->ibar.p;
+  a.u_direct = u_direct;
+  a.rho_bar = p->rho_bar_b.p;
+  a.rb_stride = p->rb_stride;
+  a.gen = p->gen_b.p;
+  a.gen_stride = (long)p->d.n_lines * p->L.V;
+  a.tau = tau;
+  a.dE = p->g.dE;
+  a.n_beams = p->d.n_beams;
+  return a;
+}
+
+// Forward pass with checkpoints, then the exact adjoint of the discrete march:
+// per segment, recompute the states after every step and after its first odd
+// half step, and back-propagate each step as three fused passes.
+int enqueue_forward_adjoint_exact(pnx_problem* p) {
+  hipStream_t s = p->ctx->stream;
+  const long st = p->L.beam_stride * p->d.n_beams;
+  const int n = p->d.n_steps, K = p->g.K, N = p->d.N, dims = p->L.dims;
+  const double dE = p->g.dE, h = dE / 2;
+  int rc;
+  HIP_TRY(launch_fill(p->state.p, st, 0.0, s));
+  for (int k = 1; k <= n; ++k) {
+    if ((k - 1) % K == 0)
+      HIP_TRY(hipMemcpyAsync(p->ckpt.p + (long)((k - 1) / K) * st, p->state.p, st * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step))) return rc;
+  }
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p,
+                                  p->g.rec_len, p->g.sigma_off, n, p->d.n_lines, dE, p->L.dvol, p->d.n_beams,
+                                  p->I_dev.p, s));
+  HIP_TRY(launch_fill(p->lam.p, st, 0.0, s));
+  HIP_TRY(launch_fill(p->w1.p, st, 0.0, s));
+  HIP_TRY(launch_fill(p->w2.p, st, 0.0, s));
+  HIP_TRY(launch_fill(p->rho_bar_b.p, (long)p->rho_bar_b.n, 0.0, s));
+  HIP_TRY(launch_fill(p->gen_b.p, (long)p->gen_b.n, 0.0, s));
+  double* segU = p->seg.p;
+  double* segM = p->seg.p + (long)K * st;
+  for (int sgi = p->g.nck - 1; sgi >= 0; --sgi) {
+    const int k0 = sgi * K, k1 = std::min(n, k0 + K);
+    const double* ck = p->ckpt.p + (long)sgi * st;
+    for (int k = k0 + 1; k <= k1; ++k) {
+      const double* src = k == k0 + 1 ? ck : segU + (long)(k - k0 - 2) * st;
+      double* dst = segU + (long)(k - k0 - 1) * st;
+      double* mid = segM + (long)(k - k0 - 1) * st;
+      const double* rec = p->rec_f.p + (long)k * p->g.rec_len;
+      HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, src, src, mid, nullptr), s));
+      HIP_TRY(launch_half_step(N, dims, 0, kForward, half_args(p, kForward, rec, dE, mid, src, dst, p->partial.p), s));
+      HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec, h, dst, mid, dst, nullptr), s));
+    }
+    for (int i = k1; i > k0; --i) {
+      const double* prev = i == k0 + 1 ? ck : segU + (long)(i - k0 - 2) * st;
+      const double* cur = segU + (long)(i - k0 - 1) * st;
+      const double* mid = segM + (long)(i - k0 - 1) * st;
+      // transposed first odd half of step i+1 and intensity terms of step i; odd half 3 of step i
+      HIP_TRY(launch_disc(N, dims, 0, 1, disc_args(p, i, cur, mid, p->w1.p, p->w2.p, h, cur), s));
+      // transposed odd half 3; even half 2 (input: even of step i-1, odd after half 1)
+      HIP_TRY(launch_disc(N, dims, 0, 0, disc_args(p, i, mid, prev, p->w2.p, p->w2.p, dE, nullptr), s));
+      // transposed even half 2; odd half 1 (input: state of step i-1)
+      HIP_TRY(launch_disc(N, dims, 1, 1, disc_args(p, i, prev, prev, p->w2.p, p->w1.p, h, nullptr), s));
+    }
+  }
+  return PNX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -558,14 +545,15 @@ int pnx_forward(pnx_problem* p, double* I_out) {
   return PNX_OK;
 }
 
-int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out) {
-  int rc = check_ready(p, true);
+static int forward_adjoint_impl(pnx_problem* p, bool exact, const double* I_bar, double* I_out,
+                                double* const rho_bar[8], double* gen_out) {
+  int rc = check_ready(p, !exact);
   if (rc) return rc;
   if (!I_bar || !I_out) return fail(PNX_ERR_ARG, "null argument");
-  if ((rc = alloc_adjoint(p))) return rc;
+  if ((rc = alloc_adjoint(p, exact))) return rc;
   hipStream_t s = p->ctx->stream;
   HIP_TRY(hipMemcpyAsync(p->ibar.p, I_bar, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyHostToDevice, s));
-  if ((rc = enqueue_forward_adjoint(p))) return rc;
+  if ((rc = exact ? enqueue_forward_adjoint_exact(p) : enqueue_forward_adjoint(p))) return rc;
   HIP_TRY(hipMemcpyAsync(I_out, p->I_dev.p, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
   const Layout& L = p->L;
   const int ne = p->d.n_elements;
@@ -576,15 +564,12 @@ int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, doub
     HIP_TRY(hipStreamSynchronize(s));
     for (int c = 0; c < 8; ++c) {
       if (!rho_bar[c]) continue;
-      const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2];
-      const bool act = class_active(c, L.dims) && p->T.n[c] > 0;
-      for (int k = 0; k < nz; ++k)
-        for (int j = 0; j < ny; ++j)
-          for (int i = 0; i < nx; ++i) {
-            const long q = i + (long)nx * (j + (long)ny * k);
-            const long pd = i + (long)L.X * (j + (long)L.Y * k);
-            for (int e = 0; e < ne; ++e) rho_bar[c][q * ne + e] = act ? h[p->rb_off[c] + e * L.V + pd] : 0.0;
-          }
+      if (p->g.active(c)) {
+        unpack_class(p->g, c, h.data() + p->g.rho_off[c], ne, rho_bar[c]);
+      } else {
+        const long np = (long)p->g.class_n[c][0] * p->g.class_n[c][1] * p->g.class_n[c][2];
+        std::fill(rho_bar[c], rho_bar[c] + np * ne, 0.0);
+      }
     }
   }
   if (gen_out) {
@@ -593,17 +578,19 @@ int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, doub
     std::vector<double> h((size_t)gs);
     HIP_TRY(hipMemcpyAsync(h.data(), p->sum_buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const int nl = p->d.n_lines;
-    for (int k = 0; k < L.Mz; ++k)
-      for (int j = 0; j < L.My; ++j)
-        for (int i = 0; i < L.Mx; ++i) {
-          const long q = i + (long)L.Mx * (j + (long)L.My * k);
-          const long pd = i + (long)L.X * (j + (long)L.Y * k);
-          for (int r = 0; r < nl; ++r) gen_out[q * nl + r] = h[r * L.V + pd];
-        }
+    unpack_lines(p->g, h.data(), gen_out);
   }
   HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;
+}
+
+int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out) {
+  return forward_adjoint_impl(p, false, I_bar, I_out, rho_bar, gen_out);
+}
+
+int pnx_forward_adjoint_exact(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8],
+                              double* gen_out) {
+  return forward_adjoint_impl(p, true, I_bar, I_out, rho_bar, gen_out);
 }
 
 int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_bar, double* const u[8]) {
@@ -612,20 +599,12 @@ int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_
   if (!u || beam < 0 || beam >= p->d.n_beams || record < 0 || record > p->d.n_steps)
     return fail(PNX_ERR_ARG, "bad argument");
   const Layout& L = p->L;
-  const int dims = L.dims;
   hipStream_t s = p->ctx->stream;
   std::vector<double> h((size_t)L.beam_stride, 0.0);
   for (int c = 0; c < 8; ++c) {
-    if (!(class_active(c, dims) && p->T.n[c] > 0)) continue;
+    if (!p->g.active(c)) continue;
     if (!u[c]) return fail(PNX_ERR_ARG, "state missing for an updated class");
-    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2], nm = p->T.n[c];
-    for (int k = 0; k < nz; ++k)
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const long q = i + (long)nx * (j + (long)ny * k);
-          const long pd = i + (long)L.X * (j + (long)L.Y * k);
-          for (int m = 0; m < nm; ++m) h[L.cls_off[c] + m * L.V + pd] = u[c][q * nm + m];
-        }
+    pack_class(p->g, c, u[c], p->T.n[c], h.data() + L.cls_off[c]);
   }
   double* sb = p->state.p + (long)beam * L.beam_stride;
   HIP_TRY(hipMemcpyAsync(sb, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -641,17 +620,8 @@ int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_
   }
   HIP_TRY(hipMemcpyAsync(h.data(), sb, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  for (int c = 0; c < 8; ++c) {
-    if (!(class_active(c, dims) && p->T.n[c] > 0) || !u[c]) continue;
-    const int nx = p->class_n[c][0], ny = p->class_n[c][1], nz = p->class_n[c][2], nm = p->T.n[c];
-    for (int k = 0; k < nz; ++k)
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const long q = i + (long)nx * (j + (long)ny * k);
-          const long pd = i + (long)L.X * (j + (long)L.Y * k);
-          for (int m = 0; m < nm; ++m) u[c][q * nm + m] = h[L.cls_off[c] + m * L.V + pd];
-        }
-  }
+  for (int c = 0; c < 8; ++c)
+    if (p->g.active(c) && u[c]) unpack_class(p->g, c, h.data() + L.cls_off[c], p->T.n[c], u[c]);
   return PNX_OK;
 }
 
@@ -730,10 +700,10 @@ int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, do
   if (bytes_out) *bytes_out = bytes;
   if (!ms_out) return PNX_OK;
   hipStream_t s = p->ctx->stream;
-  const double* rec = p->rec_f.p + p->rec_len;
-  const double dt = parity == 0 ? p->dE : p->dE / 2;
+  const double* rec = p->rec_f.p + p->g.rec_len;
+  const double dt = parity == 0 ? p->g.dE : p->g.dE / 2;
   HalfStepArgs a = half_args(p, kForward, rec, dt, p->state.p, p->state.p, p->state.p,
-                             parity == 0 ? p->partial.p + p->partial_step : nullptr);
+                             parity == 0 ? p->partial.p + p->g.partial_step : nullptr);
   HIP_TRY(launch_half_step(p->d.N, L.dims, parity, kForward, a, s));  // warm-up
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));

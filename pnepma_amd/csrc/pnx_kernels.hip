@@ -18,6 +18,33 @@ hipError_t dens_n7(int dims, const DensArgs& a, hipStream_t s);
 hipError_t half_step_n9(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t dens_n9(int dims, const DensArgs& a, hipStream_t s);
 
+hipError_t half_step_grouped_n1(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s);
+hipError_t dens_grouped_n1(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t half_step_grouped_n3(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s);
+hipError_t dens_grouped_n3(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t half_step_grouped_n5(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s);
+hipError_t dens_grouped_n5(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t half_step_grouped_n7(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s);
+hipError_t dens_grouped_n7(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t half_step_grouped_n9(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s);
+hipError_t dens_grouped_n9(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+
+hipError_t disc_n1(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+hipError_t disc_n3(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+hipError_t disc_n5(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+hipError_t disc_n7(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+hipError_t disc_n9(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
   switch (N) {
     case 1: return half_step_n1(dims, parity, mode, a, s);
@@ -36,6 +63,41 @@ hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s) {
     case 5: return dens_n5(dims, a, s);
     case 7: return dens_n7(dims, a, s);
     case 9: return dens_n9(dims, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
+                                    const int2* items, const int* n_items, int grid, hipStream_t s) {
+  switch (N) {
+    case 1: return half_step_grouped_n1(dims, parity, mode, args, items, n_items, grid, s);
+    case 3: return half_step_grouped_n3(dims, parity, mode, args, items, n_items, grid, s);
+    case 5: return half_step_grouped_n5(dims, parity, mode, args, items, n_items, grid, s);
+    case 7: return half_step_grouped_n7(dims, parity, mode, args, items, n_items, grid, s);
+    case 9: return half_step_grouped_n9(dims, parity, mode, args, items, n_items, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                               hipStream_t s) {
+  switch (N) {
+    case 1: return dens_grouped_n1(dims, args, items, n_items, grid, s);
+    case 3: return dens_grouped_n3(dims, args, items, n_items, grid, s);
+    case 5: return dens_grouped_n5(dims, args, items, n_items, grid, s);
+    case 7: return dens_grouped_n7(dims, args, items, n_items, grid, s);
+    case 9: return dens_grouped_n9(dims, args, items, n_items, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return disc_n1(dims, pb, pa, a, s);
+    case 3: return disc_n3(dims, pb, pa, a, s);
+    case 5: return disc_n5(dims, pb, pa, a, s);
+    case 7: return disc_n7(dims, pb, pa, a, s);
+    case 9: return disc_n9(dims, pb, pa, a, s);
   }
   return hipErrorInvalidValue;
 }

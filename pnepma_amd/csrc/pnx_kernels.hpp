@@ -72,9 +72,50 @@ struct DensArgs {
   int n_beams;
 };
 
+// Exact adjoint of one forward energy step, one fused pass per half step.
+// In the cell of every thread the pass first applies the transposed stencil of
+// the half step that was back-propagated last (parity PB points, reading its
+// w at the neighbouring points of the other parity), then back-propagates the
+// half step of parity PA at this cell's parity-PA points: w = lam tau phi / S
+// is stored for the next pass, lam is multiplied through, and the density
+// gradient of the half step is accumulated.
+struct DiscArgs {
+  Layout L;
+  const double* s_from;   // forward input state of the PA half step, other parity (beam 0)
+  const double* s_own;    // forward input state of the PA half step, parity PA (beam 0)
+  double* lam;            // adjoint state (beam 0)
+  const double* w_in;     // w of the PB pass (beam 0)
+  double* w_out;          // w of the PA half step (beam 0)
+  const double* rho;
+  long rho_off[8];
+  const double* coef;     // forward coefficients
+  const double* rec;      // forward record of the step
+  const double* prof;
+  long prof_stride;
+  const double* mms;
+  const double* W;        // [r][V]
+  const double* ibar;     // [b][n_lines]
+  const double* u_direct; // state after the step (eee zeroth moment) for the intensity terms, or null
+  double* rho_bar;        // [b][c][e][V]
+  long rb_stride;
+  double* gen;            // [b][r][V]
+  long gen_stride;
+  double tau;
+  double dE;
+  int n_beams;
+};
+
 // Launchers (defined in pnx_kernels.hip); return hipSuccess or the launch error.
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s);
+// Persistent forms over a device work list of (argument index, tile) items
+// with *n_items entries; `grid` blocks stride over the list.
+hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
+                                    const int2* items, const int* n_items, int grid, hipStream_t s);
+hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
+// pb, pa in {-1 (none), 0 (even), 1 (odd)}
+hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps
 hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,

@@ -207,10 +207,8 @@ __device__ __forceinline__ void block_coords(int n_tiles, int n_beams, int& beam
 }
 
 template <int N, int DIM, int PAR, int MODE>
-__global__ __launch_bounds__(kBlock) void half_step_kernel(HalfStepArgs a) {
+__device__ __forceinline__ void half_step_body(const HalfStepArgs& a, int beam, int tile) {
   const Layout& L = a.L;
-  int beam, tile;
-  block_coords(L.n_tiles, a.n_beams, beam, tile);
   const int q = tile * kBlock + threadIdx.x;
   const bool valid = q < L.n_cells;
   int i = 0, j = 0, k = 0;
@@ -250,6 +248,27 @@ __global__ __launch_bounds__(kBlock) void half_step_kernel(HalfStepArgs a) {
       for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
       a.partial[beam * a.partial_stride + (long)tile * L.n_lines + threadIdx.x] = s;
     }
+    __syncthreads();
+  }
+}
+
+template <int N, int DIM, int PAR, int MODE>
+__global__ __launch_bounds__(kBlock) void half_step_kernel(HalfStepArgs a) {
+  int beam, tile;
+  block_coords(a.L.n_tiles, a.n_beams, beam, tile);
+  half_step_body<N, DIM, PAR, MODE>(a, beam, tile);
+}
+
+// Persistent form for heterogeneous batches: blocks stride over a work list of
+// (argument block, tile) items, each argument block describing one request.
+template <int N, int DIM, int PAR, int MODE>
+__global__ __launch_bounds__(kBlock) void half_step_grouped(const HalfStepArgs* __restrict__ args,
+                                                            const int2* __restrict__ items,
+                                                            const int* __restrict__ n_items) {
+  const int n = *n_items;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    half_step_body<N, DIM, PAR, MODE>(args[w.x], 0, w.y);
   }
 }
 
@@ -309,10 +328,8 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, const double* __re
 }
 
 template <int N, int DIM>
-__global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
+__device__ __forceinline__ void dens_body(const DensArgs& a, int beam, int tile) {
   const Layout& L = a.L;
-  int beam, tile;
-  block_coords(L.n_tiles, a.n_beams, beam, tile);
   const int q = tile * kBlock + threadIdx.x;
   if (q >= L.n_cells) return;
   const int i = q % L.Mx;
@@ -331,6 +348,255 @@ __global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
   });
 }
 
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
+  int beam, tile;
+  block_coords(a.L.n_tiles, a.n_beams, beam, tile);
+  dens_body<N, DIM>(a, beam, tile);
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void dens_grouped(const DensArgs* __restrict__ args, const int2* __restrict__ items,
+                                                       const int* __restrict__ n_items) {
+  const int n = *n_items;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    dens_body<N, DIM>(args[w.x], 0, w.y);
+  }
+}
+
+// ----------------------------------------------------------- exact adjoint
+__device__ __forceinline__ double dexpm1div(double x) {
+  // derivative of expm1div: Taylor branch as the forward, else (e^x (x - 1) + 1) / x^2
+  // (series below |x| < 0.05 to avoid the cancellation)
+  const double ax = fabs(x);
+  if (ax <= 0.0002) return 0.5 + x / 3.0;
+  if (ax < 0.05) {
+    double term = 1.0, sum = 0.0, fact = 1.0, xp = 1.0;
+#pragma unroll
+    for (int n = 1; n < 12; ++n) {
+      fact *= (n + 1);
+      sum += n * xp / fact;
+      xp *= x;
+    }
+    (void)term;
+    return sum;
+  }
+  return (exp(x) * (x - 1.0) + 1.0) / (x * x);
+}
+
+// acc[NF] += sign/h * A_D[C]^T w  for one neighbour point of class C
+template <int N, int C, int D>
+__device__ __forceinline__ void at_w(const double* __restrict__ coef, const double* __restrict__ wc, long V, long q,
+                                     double scale, double* acc) {
+  constexpr int NC = ncls<N, C>();
+  constexpr int F = switch_axis(C, D);
+  constexpr int NF = ncls<N, F>();
+  double w[NC];
+#pragma unroll
+  for (int I = 0; I < NC; ++I) w[I] = wc[I * V + q];
+  static_for<0, NF>([&](auto Jc) {
+    constexpr int J = decltype(Jc)::value;
+    double s = 0.0;
+    static_for<0, NC>([&](auto Ic) {
+      constexpr int I = decltype(Ic)::value;
+      constexpr int z = Tab<N>::t.nz[C][D][I][J];
+      if constexpr (z >= 0) s += coef[z] * w[I];
+    });
+    acc[J] += scale * s;
+  });
+}
+
+// Transposed stencil into lam of class F at point p from the w of the classes
+// that read F (switch_D(F) along every non-singleton axis D).
+template <int N, int DIM, int F>
+__device__ __forceinline__ void disc_b(const DiscArgs& a, int beam, int i, int j, int k, long p) {
+  constexpr int NF = ncls<N, F>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  const long bo = (long)beam * L.beam_stride;
+  double acc[NF];
+#pragma unroll
+  for (int J = 0; J < NF; ++J) acc[J] = 0.0;
+  static_for<0, DIM>([&](auto Dc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr int C = switch_axis(F, D);
+    if constexpr (ncls<N, C>() > 0) {
+      const long st = D == 0 ? 1L : (D == 1 ? (long)L.X : (long)L.X * L.Y);
+      const int iq = D == 0 ? i : (D == 1 ? j : k);
+      const int M = D == 0 ? L.Mx : (D == 1 ? L.My : L.Mz);
+      const double ih = L.inv_h[D];
+      const double* __restrict__ wc = a.w_in + bo + L.cls_off[C];
+      if constexpr (odd_in(C, D)) {  // F even along D: read by C at iq (as R) and iq+1 (as L)
+        if (iq >= 1) at_w<N, C, D>(a.coef, wc, V, p, -ih, acc);
+        if (iq + 1 <= M - 1) at_w<N, C, D>(a.coef, wc, V, p + st, ih, acc);
+      } else {  // F odd along D (iq in 1..M-1): read by C at iq (as L) and iq-1 (as R); ghosts double
+        const double cl = iq == M - 1 ? 2.0 : 1.0, cr = iq == 1 ? 2.0 : 1.0;
+        at_w<N, C, D>(a.coef, wc, V, p, cl * ih, acc);
+        at_w<N, C, D>(a.coef, wc, V, p - st, -cr * ih, acc);
+      }
+    }
+  });
+  double* __restrict__ lf = a.lam + bo + L.cls_off[F];
+#pragma unroll
+  for (int J = 0; J < NF; ++J) lf[J * V + p] += acc[J];
+}
+
+// Back-propagation of the half step of class C at point p.
+template <int N, int DIM, int C>
+__device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j, int k, long p) {
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  const int ne = L.n_e;
+  const long bo = (long)beam * L.beam_stride;
+  const double* __restrict__ uc = a.s_own + bo + L.cls_off[C];
+  double u[NC], d[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) {
+    u[m] = uc[m * V + p];
+    d[m] = 0.0;
+  }
+  const double gfac = a.rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, true>(L, a.s_from + bo, a.coef, u, d, i, j, k, p, gfac, a.prof + beam * a.prof_stride, a.mms);
+  double S, Cl[N + 1];
+  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
+  double* lc = a.lam + bo + L.cls_off[C];
+  double lam[NC], w[NC], dS[NC], dC[NC];
+  const double tau = a.tau;
+  static_for<0, NC>([&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    constexpr int l = Tab<N>::t.l[C][I];
+    const double x = -Cl[l] / S * tau;
+    const double ph = expm1div(x), dph = dexpm1div(x);
+    const double r = (-d[I]) - Cl[l] * u[I];
+    const double yb = lc[I * V + p];
+    w[I] = yb * tau * ph / S;
+    dS[I] = yb * (tau * r / (S * S)) * (-x * dph - ph);
+    dC[I] = yb * (-(tau / S) * (dph * tau * r / S + ph * u[I]));
+    lam[I] = yb - w[I] * Cl[l];
+  });
+  // own values reach the stencil through the boundary ghosts of the faces
+  static_for<0, DIM>([&](auto Dc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr int F = switch_axis(C, D);
+    constexpr int NF = ncls<N, F>();
+    if constexpr (!odd_in(C, D) && NF > 0) {
+      const int idx = D == 0 ? i : (D == 1 ? j : k);
+      const int M = D == 0 ? L.Mx : (D == 1 ? L.My : L.Mz);
+      if (idx == 0 || idx == M - 1) {
+        double at[NF];
+#pragma unroll
+        for (int J = 0; J < NF; ++J) at[J] = 0.0;
+        static_for<0, NF>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value;
+          static_for<0, NC>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            constexpr int z = Tab<N>::t.nz[C][D][I][J];
+            if constexpr (z >= 0) at[J] += a.coef[z] * w[I];
+          });
+        });
+        constexpr int boff = Tab<N>::t.b_off[F][D];
+        const double s2 = 2.0 * L.inv_h[D];
+#pragma unroll
+        for (int I = 0; I < NC; ++I) {
+          double bt = 0.0;
+#pragma unroll
+          for (int J = 0; J < NF; ++J) bt += a.coef[boff + J * NC + I] * at[J];
+          lam[I] -= s2 * bt;
+        }
+      }
+    }
+  });
+  double* __restrict__ wo = a.w_out + bo + L.cls_off[C];
+#pragma unroll
+  for (int I = 0; I < NC; ++I) {
+    lc[I * V + p] = lam[I];
+    wo[I * V + p] = w[I];
+  }
+  double* __restrict__ rb = a.rho_bar + beam * a.rb_stride + a.rho_off[C];
+  for (int e = 0; e < ne; ++e) {
+    const double se = a.rec[e];
+    const double* __restrict__ ce = a.rec + ne + e * (N + 1);
+    double acc = 0.0;
+    static_for<0, NC>([&](auto Ic) {
+      constexpr int I = decltype(Ic)::value;
+      constexpr int l = Tab<N>::t.l[C][I];
+      acc += dS[I] * se + dC[I] * ce[l];
+    });
+    rb[e * V + p] += acc;
+  }
+}
+
+template <int N, int DIM, int PB, int PA>
+__device__ __forceinline__ void disc_body(const DiscArgs& a, int beam, int tile) {
+  const Layout& L = a.L;
+  const int q = tile * kBlock + threadIdx.x;
+  if (q >= L.n_cells) return;
+  const int i = q % L.Mx;
+  const int r = q / L.Mx;
+  const int j = r % L.My;
+  const int k = r / L.My;
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (PB >= 0 && parity_of(C) == PB && class_active(C, DIM) && ncls<N, C>() > 0) {
+      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                              (DIM >= 3 && odd_in(C, 2) && k == 0));
+      if (interior) disc_b<N, DIM, C>(a, beam, i, j, k, p);
+    }
+  });
+  if (a.u_direct != nullptr) {
+    // d(sum_r I_bar_r I_r)/du_eee(p,0) of this step and the line-weight sensitivity
+    const long V = L.V;
+    const long bo = (long)beam * L.beam_stride;
+    const int ne = L.n_e;
+    const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
+    const double* __restrict__ ib = a.ibar + beam * L.n_lines;
+    const double u0 = a.u_direct[bo + L.cls_off[0] + p];
+    double add = 0.0;
+    double* __restrict__ g = a.gen + beam * a.gen_stride;
+    for (int rr = 0; rr < L.n_lines; ++rr) {
+      const double wk = a.dE * L.dvol * sig[rr] * ib[rr];
+      add += wk * a.W[rr * V + p];
+      g[rr * V + p] += u0 * wk;
+    }
+    a.lam[bo + L.cls_off[0] + p] += add;
+  }
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (PA >= 0 && parity_of(C) == PA && class_active(C, DIM) && ncls<N, C>() > 0) {
+      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                              (DIM >= 3 && odd_in(C, 2) && k == 0));
+      if (interior) disc_a<N, DIM, C>(a, beam, i, j, k, p);
+    }
+  });
+}
+
+template <int N, int DIM, int PB, int PA>
+__global__ __launch_bounds__(kBlock) void disc_kernel(DiscArgs a) {
+  int beam, tile;
+  block_coords(a.L.n_tiles, a.n_beams, beam, tile);
+  disc_body<N, DIM, PB, PA>(a, beam, tile);
+}
+
+template <int N>
+hipError_t disc_order(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s) {
+  const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
+  // the three passes used by the sweep: (even, odd), (even, even), (odd, odd)
+  const int key = dims * 16 + (pb + 1) * 4 + (pa + 1);
+  switch (key) {
+#define PNX_DCASE(D, PB_, PA_) \
+  case D * 16 + (PB_ + 1) * 4 + (PA_ + 1): hipLaunchKernelGGL((disc_kernel<N, D, PB_, PA_>), grid, block, 0, s, a); break;
+    PNX_DCASE(1, 0, 1) PNX_DCASE(1, 0, 0) PNX_DCASE(1, 1, 1)
+    PNX_DCASE(2, 0, 1) PNX_DCASE(2, 0, 0) PNX_DCASE(2, 1, 1)
+    PNX_DCASE(3, 0, 1) PNX_DCASE(3, 0, 0) PNX_DCASE(3, 1, 1)
+#undef PNX_DCASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int N>
 hipError_t half_step_order(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
   const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
@@ -341,6 +607,35 @@ hipError_t half_step_order(int dims, int parity, int mode, const HalfStepArgs& a
     PNX_CASE(2, 0, 0) PNX_CASE(2, 0, 1) PNX_CASE(2, 1, 0) PNX_CASE(2, 1, 1)
     PNX_CASE(3, 0, 0) PNX_CASE(3, 0, 1) PNX_CASE(3, 1, 0) PNX_CASE(3, 1, 1)
 #undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t half_step_grouped_order(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
+                                   const int* n_items, int grid, hipStream_t s) {
+  switch (dims * 4 + parity * 2 + mode) {
+#define PNX_CASE(D, P, M)                                                                                  \
+  case D * 4 + P * 2 + M:                                                                                  \
+    hipLaunchKernelGGL((half_step_grouped<N, D, P, M>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); \
+    break;
+    PNX_CASE(1, 0, 0) PNX_CASE(1, 0, 1) PNX_CASE(1, 1, 0) PNX_CASE(1, 1, 1)
+    PNX_CASE(2, 0, 0) PNX_CASE(2, 0, 1) PNX_CASE(2, 1, 0) PNX_CASE(2, 1, 1)
+    PNX_CASE(3, 0, 0) PNX_CASE(3, 0, 1) PNX_CASE(3, 1, 0) PNX_CASE(3, 1, 1)
+#undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t dens_grouped_order(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
+                              hipStream_t s) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((dens_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 2: hipLaunchKernelGGL((dens_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 3: hipLaunchKernelGGL((dens_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

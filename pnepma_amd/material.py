@@ -62,6 +62,31 @@ class CellFractions:
     def with_params(self, p) -> "CellFractions":
         return CellFractions(np.asarray(p, dtype=np.float64).reshape(self.vals.shape), *self.edges, self.embedding)
 
+    def cells_uniform(self, pts):
+        """Cell indices by arithmetic on uniformly spaced edges (for generic points
+        such as absorption ray samples, which do not sit on edges)."""
+        idx = []
+        inside = np.ones(pts.shape[0], dtype=bool)
+        for d, e in enumerate(self.edges):
+            n = len(e) - 1
+            i = np.floor((pts[:, d] - e[0]) * (n / (e[-1] - e[0]))).astype(np.int64)
+            inside &= (i >= 0) & (i < n)
+            idx.append(np.clip(i, 0, n - 1))
+        return idx, inside
+
+    def linear_cells(self, pts):
+        """Flat cell index of each generic point, -1 outside the boxes."""
+        (ix, iy, iz), inside = self.cells_uniform(pts)
+        nx, ny, nz = self.vals.shape[:3]
+        lin = (ix * ny + iy) * nz + iz
+        return np.where(inside, lin, -1)
+
+    def fractions_fast(self, pts: np.ndarray) -> np.ndarray:
+        (ix, iy, iz), inside = self.cells_uniform(pts)
+        w = self.vals[ix, iy, iz]
+        w[~inside] = self.embedding
+        return w
+
     def _cells(self, pts):
         idx = []
         inside = np.ones(pts.shape[0], dtype=bool)
@@ -99,9 +124,11 @@ class MassFractionMaterial:
     def with_params(self, p) -> "MassFractionMaterial":
         return MassFractionMaterial(self.elements, self.param.with_params(p))
 
-    def densities(self, pts: np.ndarray) -> np.ndarray:
-        """component_densities! at each point: shape (P, n_e)."""
-        w = self.param.fractions(pts)
+    def densities(self, pts: np.ndarray, fast: bool = False) -> np.ndarray:
+        """component_densities! at each point: shape (P, n_e).  fast=True uses the
+        parametrization's arithmetic lookup where it has one (generic points only)."""
+        w = self.param.fractions_fast(pts) if fast and hasattr(self.param, "fractions_fast") \
+            else self.param.fractions(pts)
         R = (w / self.rho_pure).sum(axis=1, keepdims=True)
         return w / R
 

@@ -134,13 +134,33 @@ class StoppingPower:
 _GL_NODES, _GL_WEIGHTS = np.polynomial.legendre.leggauss(4000)
 
 
+_TC_GRID = np.geomspace(50.0, 1.0e5, 4097)
+_TC_CACHE = {}
+
+
 def transport_coefficients(el: Element, E, N: int) -> np.ndarray:
     """tc[l](E), l = 0..N, in cm^2/g: Legendre moments 2 pi int P_l(mu) dsigma/dOmega dmu
     of a screened-Rutherford cross section (screening 3.4 Z^(2/3) / E[eV]) per gram.
 
-    Stand-in for the reference's missing transport_coefficient.jl.  Returns an
-    array of shape E.shape + (N + 1,).
+    Stand-in for the reference's missing transport_coefficient.jl.  The moments
+    are tabulated once per element on 4097 log-spaced energies (50 eV - 100 keV)
+    and interpolated linearly in log E.  Returns an array of shape E.shape + (N + 1,).
     """
+    if N > 9:
+        return _transport_coefficients_direct(el, E, N)
+    tab = _TC_CACHE.get(el.symbol)
+    if tab is None:
+        tab = _transport_coefficients_direct(el, _TC_GRID, 9)
+        _TC_CACHE[el.symbol] = tab
+    E = np.asarray(E, dtype=np.float64)
+    x = np.log(np.clip(E, _TC_GRID[0], _TC_GRID[-1]))
+    lg = np.log(_TC_GRID)
+    i = np.clip(np.searchsorted(lg, x, side="right") - 1, 0, len(lg) - 2)
+    t = ((x - lg[i]) / (lg[i + 1] - lg[i]))[..., None]
+    return (1.0 - t) * tab[i, :N + 1] + t * tab[i + 1, :N + 1]
+
+
+def _transport_coefficients_direct(el: Element, E, N: int) -> np.ndarray:
     E = np.atleast_1d(np.asarray(E, dtype=np.float64))
     eta = 3.4 * el.Z ** (2.0 / 3.0) / E
     pref = el.Z * (el.Z + 1) * E2_EV_CM ** 2 / (4.0 * E ** 2)
@@ -153,15 +173,16 @@ def transport_coefficients(el: Element, E, N: int) -> np.ndarray:
 
 # ------------------------------------------------------------- cross sections
 def ionisation_cross_section(E, xray: XRay) -> np.ndarray:
-    """K-shell ionisation cross section in cm^2 (Bethe form, Powell's constants).
+    """K-shell ionisation cross section in cm^2: Bethe form
+    6.51e-14 n_K b_K ln(U) / (U E_K^2) with n_K = 2, b_K = 0.9, U = E / E_K.
 
     Stand-in for NeXLCore.ionizationcrosssection(..., Bote2009); zero below the edge.
     """
     E = np.asarray(E, dtype=np.float64)
     U = E / xray.edge
     with np.errstate(divide="ignore", invalid="ignore"):
-        q = 6.51e-14 * 2.0 * 0.9 * np.log(0.65 * np.maximum(U, 1e-300)) / (U * xray.edge ** 2)
-    return np.where(U > 1.0 / 0.65, q, 0.0)
+        q = 6.51e-14 * 2.0 * 0.9 * np.log(np.maximum(U, 1e-300)) / (U * xray.edge ** 2)
+    return np.where(U > 1.0, q, 0.0)
 
 
 # illustrative mass absorption coefficients (cm^2/g) of the K-L3 line of the

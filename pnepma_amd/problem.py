@@ -203,10 +203,19 @@ def absorption_factors(material: MassFractionMaterial, x_rays, gspec: GridSpec, 
     dx = np.linalg.norm(d, axis=1) * lam / 100.0
     acc = np.zeros((pts.shape[0], len(x_rays)))
     ts = np.linspace(0.0, 1.0, 100)
-    for t in ts:
-        q = pts + (t * lam)[:, None] * d
-        rho = material.densities(q)
-        acc -= rho @ macs.T
+    param = material.param
+    if hasattr(param, "linear_cells"):
+        # per-box attenuation sum_e mac[r][e] rho_e, looked up per ray sample
+        w = np.concatenate([param.vals.reshape(-1, param.vals.shape[-1]), param.embedding[None, :]])
+        rho = w / (w / material.rho_pure).sum(axis=1, keepdims=True)
+        mu = rho @ macs.T
+        for t in ts:
+            q = pts + (t * lam)[:, None] * d
+            acc -= mu[param.linear_cells(q)]  # index -1 selects the embedding row
+    else:
+        for t in ts:
+            q = pts + (t * lam)[:, None] * d
+            acc -= material.densities(q) @ macs.T
     return np.exp(acc * dx[:, None]), (pts, d, lam, dx, macs)
 
 

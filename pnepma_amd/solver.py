@@ -124,8 +124,11 @@ class DeviceProblem:
         check(lib().pnx_forward(self.handle, dptr(out)))
         return out
 
-    def forward_adjoint(self, I_bar):
-        """Returns (I, rho_bar[8] per class (P_c, n_e) summed over beams, gen (P_eee, n_lines))."""
+    def forward_adjoint(self, I_bar, exact: bool = False):
+        """Returns (I, rho_bar[8] per class (P_c, n_e) summed over beams, gen (P_eee, n_lines)).
+
+        exact=False runs the reference's adjoint; exact=True the exact gradient of the
+        discrete forward march."""
         p = self.problem
         ib = np.ascontiguousarray(np.broadcast_to(np.asarray(I_bar, dtype=np.float64),
                                                   (p.n_beams, len(p.x_rays))))
@@ -134,7 +137,8 @@ class DeviceProblem:
         rb = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)]
         gen = np.zeros((int(np.prod(p.gspec.class_size(0))), len(p.x_rays)))
         r8 = (_lib._dp * 8)(*[dptr(r) for r in rb])
-        check(lib().pnx_forward_adjoint(self.handle, dptr(ib), dptr(I), r8, dptr(gen)))
+        fn = lib().pnx_forward_adjoint_exact if exact else lib().pnx_forward_adjoint
+        check(fn(self.handle, dptr(ib), dptr(I), r8, dptr(gen)))
         return I, rb, gen
 
     def step(self, u, record: int, beam: int = 0, adjoint: bool = False, I_bar=None):

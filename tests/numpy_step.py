@@ -207,3 +207,170 @@ def pullback(p, si, adv, bnd, I_bar, beam=0):
     densities(tape[0], dens[0], 0.5)
     intensity_pb(tape[0], dens[0], 0.5)
     return I, [r.reshape(-1, ne) for r in rb], acb, noab
+
+
+# ----------------------------------------------------------------------------
+# Exact (discrete) adjoint of the forward march, dense numpy formulation.
+def dphi1(x):
+    """Derivative of the forward's phi1 (Taylor branch |x| <= 2e-4, else (e^x - 1)/x)."""
+    x = np.asarray(x, dtype=np.float64)
+    small = np.abs(x) <= 0.0002
+    mid = (~small) & (np.abs(x) < 0.05)
+    ser = np.zeros_like(x)
+    fact = 1.0
+    for n in range(1, 12):  # sum_n n x^(n-1) / (n+1)!
+        fact *= (n + 1)
+        ser = ser + n * x ** (n - 1) / fact
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        big = (np.exp(x) * (x - 1.0) + 1.0) / (x * x)
+    return np.where(small, 0.5 + x / 3.0, np.where(mid, ser, big))
+
+
+def _combine(F, even_src, odd_src):
+    return [ (even_src[c] if G.parity_of(c) == 0 else odd_src[c]).copy() for c in range(8)]
+
+
+def _inner(F, c):
+    n = F.size[c]
+    rng = []
+    for a in range(3):
+        m = n[2 - a]
+        rng.append(slice(1, m - 1) if G.odd_in(c, a) else slice(0, m))
+    return (rng[2], rng[1], rng[0])
+
+
+def _adjoint_half(F, lam, s, P, tau, rb, rec):
+    """Backpropagate lam through the half step of parity P applied to state s."""
+    F.set_record(rec)
+    s = [x.copy() for x in s]
+    F.boundary(s, 1 - P)          # ghosts of the from-classes, as the forward half step saw them
+    ne = len(F.p.elements)
+    ubar = [np.zeros_like(x) for x in s]
+    for c in range(8):
+        if G.parity_of(c) != P or not F.cm[c]:
+            continue
+        inner = _inner(F, c)
+        n = F.size[c]
+        S = F.S[c][inner]
+        Cm = F.C[c][inner]
+        u = s[c][inner]
+        # r = -d - C u with d from s (ghosts included)
+        d = np.zeros(u.shape)
+        for a in range(3):
+            if n[2 - a] == 1:
+                continue
+            f = G.switch_axis(c, a)
+            if not F.cm[f]:
+                continue
+            R, L = list(inner), list(inner)
+            r_ = inner[2 - a]
+            if G.odd_in(c, a):
+                R[2 - a], L[2 - a] = slice(r_.start, r_.stop), slice(r_.start - 1, r_.stop - 1)
+            else:
+                R[2 - a], L[2 - a] = slice(r_.start + 1, r_.stop + 1), slice(r_.start, r_.stop)
+            d = d + ((s[f][tuple(R)] - s[f][tuple(L)]) / F.h[a]) @ F.A[c][a].T
+        r = -d - Cm * u
+        x = -Cm / S * tau
+        ph = phi1(x)
+        yb = lam[c][inner]
+        w = yb * tau * ph / S
+        # densities: y = u + tau phi(x) r / S, x = -C tau / S
+        dS = (tau * r / S ** 2) * (-x * dphi1(x) - ph)
+        dC = -(tau / S) * (dphi1(x) * tau * r / S + ph * u)
+        ls = [l for l, _ in F.cm[c]]
+        for e in range(ne):
+            rb[c][inner + (e,)] += (yb * (dS * F.s[e] + dC * F.cl[e, ls])).sum(axis=-1)
+        # own values
+        ubar[c][inner] += yb - w * Cm
+        # from-classes through the stencil (scatter form, ghosts included)
+        for a in range(3):
+            if n[2 - a] == 1:
+                continue
+            f = G.switch_axis(c, a)
+            if not F.cm[f]:
+                continue
+            gbar = -(w @ F.A[c][a]) / F.h[a]       # d(y)/d(delta) contribution
+            R, L = list(inner), list(inner)
+            r_ = inner[2 - a]
+            if G.odd_in(c, a):
+                R[2 - a], L[2 - a] = slice(r_.start, r_.stop), slice(r_.start - 1, r_.stop - 1)
+            else:
+                R[2 - a], L[2 - a] = slice(r_.start + 1, r_.stop + 1), slice(r_.start, r_.stop)
+            ubar[f][tuple(R)] += gbar
+            ubar[f][tuple(L)] -= gbar
+    # chain the from-classes' ghost adjoints through apply_boundary (order reversed)
+    for a in reversed(range(3)):
+        if F.g.axes[a].singleton:
+            continue
+        for f in reversed(range(8)):
+            if not (G.odd_in(f, a) and G.parity_of(f) == 1 - P) or not F.cm[f]:
+                continue
+            v = G.switch_axis(f, a)     # class of parity P (the face values)
+            Bm = F.B[(f, a)]
+            m = s[f].shape[2 - a]
+            gl = ubar[f][F._sl(a, 0)].copy()
+            gr = ubar[f][F._sl(a, m - 1)].copy()
+            ubar[f][F._sl(a, 0)] = 0.0
+            ubar[f][F._sl(a, m - 1)] = 0.0
+            ubar[f][F._sl(a, 1)] -= gl
+            ubar[f][F._sl(a, m - 2)] -= gr
+            ubar[v][F._sl(a, 0)] += -2.0 * (gl @ Bm)
+            ubar[v][F._sl(a, m - 2 if False else s[v].shape[2 - a] - 1)] += 2.0 * (gr @ Bm)
+    for c in range(8):
+        if G.parity_of(c) == P:
+            lam[c] = ubar[c]
+        else:
+            lam[c] = lam[c] + ubar[c]
+    # adjoints at ghost slots are not state
+    for c in range(8):
+        if not F.cm[c]:
+            continue
+        mask = np.zeros(F.size[c], dtype=bool)
+        mask[_inner(F, c)] = True
+        lam[c][~mask] = 0.0
+
+
+def discrete_pullback(p, si, adv, bnd, I_bar, beam=0):
+    """Exact gradient of sum_r I_bar_r I_r of the forward march: returns
+    (I, rho_bar[8] (P_c, n_e), gen (P_eee, n_lines)) with gen the sensitivity to W."""
+    fwd, _, _ = p.records()
+    ne, nl = len(p.elements), len(p.x_rays)
+    sig0 = ne + ne * (p.N + 1) + 1
+    A, Bf = dense_blocks(p.N, False, adv, bnd)
+    F = DenseProblem(p, si, A, Bf, False, beam)
+    dE, dvol = p.dE, p.gspec.dvol
+    Wl = si.W
+    u = F.zeros()
+    tape = [[x.copy() for x in u]]
+    mids = [None]
+    I = np.zeros(nl)
+    for k in range(1, p.n_steps + 1):
+        F.set_record(fwd[k])
+        h = dE / 2
+        F.boundary(u, 0)
+        du = F.calc_du(u, 1, h, True)
+        for c, dd in du.items():
+            u[c] += h * dd
+        mids.append([x.copy() for x in u])
+        F.boundary(u, 1)
+        du = F.calc_du(u, 0, dE, True)
+        for c, dd in du.items():
+            u[c] += dE * dd
+        F.boundary(u, 0)
+        du = F.calc_du(u, 1, h, True)
+        for c, dd in du.items():
+            u[c] += h * dd
+        tape.append([x.copy() for x in u])
+        I += dE * dvol * fwd[k][sig0:] * (Wl * u[0][..., 0].ravel()[:, None]).sum(axis=0)
+    lam = F.zeros()
+    rb = [np.zeros(F.rho[c].shape) for c in range(8)]
+    gen = np.zeros((tape[0][0][..., 0].size, nl))
+    for k in range(p.n_steps, 0, -1):
+        wk = dE * dvol * fwd[k][sig0:] * I_bar
+        lam[0][..., 0] += (Wl @ wk).reshape(lam[0].shape[:3])
+        gen += tape[k][0][..., 0].ravel()[:, None] * wk[None, :]
+        even_k, mid, prev = tape[k], mids[k], tape[k - 1]
+        _adjoint_half(F, lam, _combine(F, even_k, mid), 1, dE / 2, rb, fwd[k])
+        _adjoint_half(F, lam, _combine(F, prev, mid), 0, dE, rb, fwd[k])
+        _adjoint_half(F, lam, prev, 1, dE / 2, rb, fwd[k])
+    return I, [r.reshape(-1, ne) for r in rb], gen

@@ -1,0 +1,70 @@
+"""GPU serving: every result of the continuous-batching server equals the same
+request solved alone (pnx_forward_adjoint with one beam), and matches the
+oracle, while requests of mixed PN order, dimension and size share the device
+under an HBM budget that forces queueing."""
+import math
+
+import numpy as np
+import pytest
+
+from pnepma_amd import serving
+from pnepma_amd.problem import solve_inputs
+from pnepma_amd.solver import DeviceProblem
+
+from .helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs(n=10, seed=11):
+    return serving.serving_requests(n, seed=seed, sizes_1d=(30, 70), sizes_2d=(10, 18), max_steps=40)
+
+
+def _alone(spec):
+    si = solve_inputs(spec.problem, spec.material, spec.detector)
+    K = max(1, math.ceil(math.sqrt(spec.problem.n_steps)))
+    dp = DeviceProblem(spec.problem, si, ckpt_every=K)
+    I, rb, gen = dp.forward_adjoint(spec.I_bar[None, :])
+    dp.close()
+    return I[0], rb, gen
+
+
+def test_served_equals_alone():
+    specs = _specs()
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    need = max(serving.request_device_bytes(s.problem) for s in specs)
+    srv = serving.Server(hbm_budget_bytes=3 * need, max_active=64)
+    tickets = []
+    for n, pr in enumerate(preps):
+        tickets.append(srv.submit(pr, arrival_s=0.002 * n))
+    for t in tickets:
+        tm = srv.wait(t)
+        assert tm.status == 0
+    st = srv.stats()
+    print("server stats", st, flush=True)
+    assert st["completed"] == len(specs)
+    assert st["peak_bytes"] <= 3 * need + 64 * 1024
+    srv.close()
+    for spec, pr in zip(specs, preps):
+        I, rb, gen = _alone(spec)
+        p = spec.problem
+        print(f"N={p.N} M={p.gspec.M} steps={p.n_steps} I={pr.I_out}", flush=True)
+        assert np.array_equal(pr.I_out, I) or rel_err(pr.I_out, I) < 1e-12
+        for c in p.gspec.active_classes():
+            assert rel_err(pr.rho_bar[c], rb[c]) < 1e-12
+        assert rel_err(pr.gen, gen) < 1e-12
+
+
+def test_served_matches_oracle(oracle_lib):
+    specs = _specs(n=4, seed=5)
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=16)
+    for t in [srv.submit(pr) for pr in preps]:
+        srv.wait(t)
+    srv.close()
+    for spec, pr in zip(specs, preps):
+        I, rb, acb, noab = oracle_lib.forward_adjoint(spec.problem, pr.si, spec.I_bar)
+        assert rel_err(pr.I_out, I) < 1e-10
+        for c in spec.problem.gspec.active_classes():
+            assert rel_err(pr.rho_bar[c], rb[c]) < 1e-8
+        assert rel_err(pr.si.noa[:, pr.si.el_idx] * pr.gen, acb) < 1e-8
