@@ -108,8 +108,11 @@ __device__ __forceinline__ void stencil(const Layout& L, const double* __restric
         const double d = (vR[J] - vL[J]) * ih;
         static_for<0, NC>([&](auto Ic) {
           constexpr int I = decltype(Ic)::value;
-          constexpr int z = Tab<N>::t.nz[C][D][I][J];
-          if constexpr (z >= 0) du[I] += coef[z] * d;
+          // advection values are compile-time constants (forward sign; the adjoint problem negates A)
+          if constexpr (Tab<N>::t.nz[C][D][I][J] >= 0) {
+            constexpr double av = SRC ? Tab<N>::t.a[C][D][I][J] : -Tab<N>::t.a[C][D][I][J];
+            du[I] += av * d;
+          }
         });
       });
     }
@@ -155,9 +158,10 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, const double
   double S, Cl[N + 1];
   mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
   double phi[N + 1];
+  const double invS = 1.0 / S;
   static_for<0, N + 1>([&](auto lc) {
     constexpr int l = decltype(lc)::value;
-    if constexpr (class_has_degree<N, C>(l)) phi[l] = expm1div(-Cl[l] / S * a.dt);
+    if constexpr (class_has_degree<N, C>(l)) phi[l] = expm1div(-Cl[l] * invS * a.dt);
   });
   double q0 = 0.0;
   if constexpr (MODE == kAdjoint && C == 0) {
@@ -179,7 +183,7 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, const double
     } else {
       t = (-du[I]) - Cl[l] * u[I];
     }
-    t = t / S;
+    t = t * invS;
     t = t * phi[l];
     const double un = u[I] + a.dt * t;
     out[I * V + p] = un;
@@ -400,8 +404,7 @@ __device__ __forceinline__ void at_w(const double* __restrict__ coef, const doub
     double s = 0.0;
     static_for<0, NC>([&](auto Ic) {
       constexpr int I = decltype(Ic)::value;
-      constexpr int z = Tab<N>::t.nz[C][D][I][J];
-      if constexpr (z >= 0) s += coef[z] * w[I];
+      if constexpr (Tab<N>::t.nz[C][D][I][J] >= 0) s += Tab<N>::t.a[C][D][I][J] * w[I];
     });
     acc[J] += scale * s;
   });
@@ -492,8 +495,7 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
           constexpr int J = decltype(Jc)::value;
           static_for<0, NC>([&](auto Ic) {
             constexpr int I = decltype(Ic)::value;
-            constexpr int z = Tab<N>::t.nz[C][D][I][J];
-            if constexpr (z >= 0) at[J] += a.coef[z] * w[I];
+            if constexpr (Tab<N>::t.nz[C][D][I][J] >= 0) at[J] += Tab<N>::t.a[C][D][I][J] * w[I];
           });
         });
         constexpr int boff = Tab<N>::t.b_off[F][D];

@@ -62,6 +62,103 @@ PNX_HD constexpr bool a_pattern(int l, int k, int lf, int kf, int d) {
   return kf == k;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time advection values.  The closed forms of pn_solver/pn_matrices.jl:24-144
+// are sqrt((integer)/(integer)) times 1/2, sqrt(2) or a sign; cx_sqrt returns the
+// correctly rounded double square root (Newton in double, then the neighbour
+// whose exact square residual is smallest), so these constants equal the
+// runtime std::sqrt values bit for bit (checked by the host tests).
+PNX_HD constexpr double cx_abs(double x) { return x < 0 ? -x : x; }
+
+// exact a*b = p + e (Dekker's product; no FMA needed)
+PNX_HD constexpr void cx_two_prod(double a, double b, double& p, double& e) {
+  p = a * b;
+  const double sa = 134217729.0 * a, ah = sa - (sa - a), al = a - ah;
+  const double sb = 134217729.0 * b, bh = sb - (sb - b), bl = b - bh;
+  e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+}
+
+PNX_HD constexpr double cx_ulp(double r) {  // r > 0, normal
+  double u = 1.0;
+  while (u <= r) u *= 2.0;
+  while (u > r) u *= 0.5;  // u = 2^floor(log2 r)
+  return u * 2.220446049250313e-16;
+}
+
+PNX_HD constexpr double cx_sqrt(double x) {
+  if (!(x > 0.0)) return 0.0;
+  double r = x > 1.0 ? x : 1.0;
+  for (int it = 0; it < 200; ++it) {
+    const double nr = 0.5 * (r + x / r);
+    if (nr == r) break;
+    r = nr;
+  }
+  double best = r, bestd = -1.0;
+  for (int k = -2; k <= 2; ++k) {
+    const double c = r + k * cx_ulp(r);
+    double pp = 0.0, ee = 0.0;
+    cx_two_prod(c, c, pp, ee);
+    const double d = cx_abs((x - pp) - ee);
+    if (bestd < 0.0 || d < bestd) {
+      bestd = d;
+      best = c;
+    }
+  }
+  return best;
+}
+
+PNX_HD constexpr double cx_rt(long a, long b) { return cx_sqrt((double)a / (double)b); }
+PNX_HD constexpr double cx_sqrt2() { return cx_sqrt(2.0); }
+PNX_HD constexpr double cx_a(int k, int l) { return cx_rt((long)(l - k + 1) * (l + k + 1), (long)(2 * l + 3) * (2 * l + 1)); }
+PNX_HD constexpr double cx_b(int k, int l) { return cx_rt((long)(l - k) * (l + k), (long)(2 * l + 1) * (2 * l - 1)); }
+PNX_HD constexpr double cx_c(int k, int l) {
+  const double v = cx_rt((long)(l + k + 1) * (l + k + 2), (long)(2 * l + 3) * (2 * l + 1));
+  return k < 0 ? 0.0 : (k > 0 ? v : v * cx_sqrt2());
+}
+PNX_HD constexpr double cx_d(int k, int l) {
+  const double v = cx_rt((long)(l - k) * (l - k - 1), (long)(2 * l + 1) * (2 * l - 1));
+  return k < 0 ? 0.0 : (k > 0 ? v : v * cx_sqrt2());
+}
+PNX_HD constexpr double cx_e(int k, int l) {
+  const double v = cx_rt((long)(l - k + 1) * (l - k + 2), (long)(2 * l + 3) * (2 * l + 1));
+  return k == 1 ? v * cx_sqrt2() : v;
+}
+PNX_HD constexpr double cx_f(int k, int l) {
+  const double v = cx_rt((long)(l + k) * (l + k - 1), (long)(2 * l + 1) * (2 * l - 1));
+  return k == 1 ? v * cx_sqrt2() : v;
+}
+PNX_HD constexpr double cx_advection(int l, int k, int lf, int kf, int d) {
+  const int ak = k < 0 ? -k : k;
+  const double th = k < 0 ? -1.0 : 1.0;
+  if (l == lf + 1) {  // A_minus
+    if (d == 0) {
+      if (kf == minus_k(k) && k != -1) return 0.5 * cx_c(ak - 1, l - 1);
+      if (kf == plus_k(k)) return -0.5 * cx_e(ak + 1, l - 1);
+      return 0.0;
+    }
+    if (d == 1) {
+      if (kf == -minus_k(k) && k != 1) return (-th / 2.0) * cx_c(ak - 1, l - 1);
+      if (kf == -plus_k(k)) return (-th / 2.0) * cx_e(ak + 1, l - 1);
+      return 0.0;
+    }
+    return kf == k ? cx_a(k, l - 1) : 0.0;
+  }
+  if (l == lf - 1) {  // A_plus
+    if (d == 0) {
+      if (kf == minus_k(k) && k != -1) return -0.5 * cx_d(ak - 1, l + 1);
+      if (kf == plus_k(k)) return 0.5 * cx_f(ak + 1, l + 1);
+      return 0.0;
+    }
+    if (d == 1) {
+      if (kf == -minus_k(k) && k != 1) return (th / 2.0) * cx_d(ak - 1, l + 1);
+      if (kf == -plus_k(k)) return (th / 2.0) * cx_f(ak + 1, l + 1);
+      return 0.0;
+    }
+    return kf == k ? cx_b(k, l + 1) : 0.0;
+  }
+  return 0.0;
+}
+
 struct Tables {
   int N;
   int n[8];                          // moments per class
@@ -70,6 +167,7 @@ struct Tables {
   int nz[8][3][kMaxClass][kMaxClass];  // index into coefficient buffer, -1 if structurally zero
   int n_a;                           // number of A values
   int b_off[8][3];                   // offset of dense boundary block B_d[c] (row-major n[c] x n[switch]), -1 if c even in d
+  double a[8][3][kMaxClass][kMaxClass];  // advection values (forward sign) at the structural nonzeros
   int n_coef;                        // total doubles in the coefficient buffer
 };
 
@@ -89,10 +187,16 @@ PNX_HD constexpr Tables make_tables(int N) {
     for (int d = 0; d < 3; ++d) {
       const int f = switch_axis(c, d);
       for (int i = 0; i < kMaxClass; ++i)
-        for (int j = 0; j < kMaxClass; ++j) t.nz[c][d][i][j] = -1;
+        for (int j = 0; j < kMaxClass; ++j) {
+          t.nz[c][d][i][j] = -1;
+          t.a[c][d][i][j] = 0.0;
+        }
       for (int j = 0; j < t.n[f]; ++j)       // column (from-moment) major: CSC order
         for (int i = 0; i < t.n[c]; ++i)
-          if (a_pattern(t.l[c][i], t.k[c][i], t.l[f][j], t.k[f][j], d)) t.nz[c][d][i][j] = idx++;
+          if (a_pattern(t.l[c][i], t.k[c][i], t.l[f][j], t.k[f][j], d)) {
+            t.nz[c][d][i][j] = idx++;
+            t.a[c][d][i][j] = cx_advection(t.l[c][i], t.k[c][i], t.l[f][j], t.k[f][j], d);
+          }
     }
   t.n_a = idx;
   for (int d = 0; d < 3; ++d)

@@ -60,9 +60,10 @@ def test_exact_adjoint_vs_central_differences(oracle_lib):
     ad = sum(float((rb[c] * dirs[c]).sum()) for c in range(8))
     print("directional", ad, fd, flush=True)
     assert abs(ad - fd) <= 1e-5 * abs(fd)
-    # line weights enter linearly
-    dW = rng.standard_normal(si.W.shape) * si.W
+    # line weights W = ac * noa[el] enter linearly: perturb the absorption factors
+    dac = rng.standard_normal(si.ac.shape) * si.ac * 1e-3
     s2 = copy.copy(si)
-    s2.W = si.W + dW
+    s2.ac = si.ac + dac
     fdW = oracle_lib.forward(p, s2) @ ib - oracle_lib.forward(p, si) @ ib
+    dW = dac * si.noa[:, si.el_idx]
     assert abs(float((gen * dW).sum()) - fdW) <= 1e-8 * abs(fdW)
