@@ -143,9 +143,14 @@ def cpu_sample(specs, preps, budget_s=20.0):
     return n, el
 
 
+def progress(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def main_serve(args):
     import numpy as np
     specs, preps = prepare_stream(args.requests)
+    progress(f"prepared {len(preps)} requests")
     import torch
     assert torch.cuda.is_available()
     torch.cuda.init()
@@ -158,6 +163,7 @@ def main_serve(args):
     head = preps[:min(len(preps), 256)]
     w0, _, _ = serve_run(head, 0.0, budget, max_active)
     capacity = len(head) / w0
+    progress(f"capacity probe: {len(head)} requests in {w0:.2f} s")
     rate = 1.2 * capacity
     for _ in range(max(0, args.warmup - 1)):
         serve_run(preps[:64], rate, budget, max_active)
@@ -168,6 +174,7 @@ def main_serve(args):
         w, lat, stats = serve_run(preps, rate, budget, max_active)
         walls.append(w)
         lats.append(lat)
+        progress(f"served pass: {len(preps)} requests in {w:.2f} s")
     torch.cuda.synchronize()
     total_wall = time.perf_counter() - t0
     lat = np.concatenate(lats)
@@ -175,9 +182,11 @@ def main_serve(args):
     # results of the served pass vs the same requests alone (plain loop)
     seq_t, seq_I = sequential_run(specs, preps)
     seq_rate = len(seq_t) / seq_t.sum()
+    progress(f"plain loop: {len(seq_t)} requests, {seq_rate:.2f} solves/s")
     # the plain-loop sample is the head of the stream: compare with the served results
     alone_err = max(float(np.max(np.abs(a - pr.I_out)) / np.max(np.abs(a))) for a, pr in zip(seq_I, preps))
     n_cpu, el_cpu = cpu_sample(specs, preps) if not args.no_cpu else (0, 0.0)
+    progress(f"cpu sample: {n_cpu} requests in {el_cpu:.1f} s")
     roof = roofline_probe()
     out = {
         "metric": METRIC, "value": served, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
