@@ -2,10 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include "pnx_kernels.hpp"
+#include "pnx_kernels_impl.hpp"
 
 namespace pnx {
-
-constexpr int kMaxLinesR = 8;
 
 hipError_t half_step_n1(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t dens_n1(int dims, const DensArgs& a, hipStream_t s);
@@ -102,12 +101,6 @@ hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipSt
   return hipErrorInvalidValue;
 }
 
-__device__ __forceinline__ double wave_sum_r(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __global__ void fill_kernel(double* __restrict__ p, long n, double v) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long stride = (long)gridDim.x * blockDim.x;
@@ -122,37 +115,15 @@ __global__ void beam_sum_kernel(const double* __restrict__ in, long stride, int 
   out[i] = s;
 }
 
-// One block per beam; thread t sums the steps k = 1 + t, 1 + t + 256, ...
+// One block per beam (the body is shared with the march kernel).
 __global__ __launch_bounds__(kBlock) void reduce_intensity_kernel(const double* __restrict__ partial, long step_stride,
                                                                   long beam_stride, int n_tiles,
                                                                   const double* __restrict__ rec, int rec_len,
                                                                   int sigma_off, int n_steps, int n_lines, double dE,
                                                                   double dvol, double* __restrict__ I_out) {
   const int b = blockIdx.x;
-  __shared__ double red[kBlock / 64][kMaxLinesR];
-  double acc[kMaxLinesR];
-#pragma unroll
-  for (int r = 0; r < kMaxLinesR; ++r) acc[r] = 0.0;
-  for (int kk = 1 + threadIdx.x; kk <= n_steps; kk += kBlock) {
-    const double* __restrict__ ps = partial + b * beam_stride + kk * step_stride;
-    const double* __restrict__ sig = rec + (long)kk * rec_len + sigma_off;
-    for (int r = 0; r < n_lines && r < kMaxLinesR; ++r) {
-      double integral = 0.0;
-      for (int t = 0; t < n_tiles; ++t) integral += ps[(long)t * n_lines + r];
-      acc[r] += dE * dvol * sig[r] * integral * 1.0;
-    }
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r = 0; r < n_lines && r < kMaxLinesR; ++r) {
-    const double s = wave_sum_r(acc[r]);
-    if (lane == 0) red[wave][r] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < n_lines && threadIdx.x < kMaxLinesR) {
-    double s = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-    I_out[b * n_lines + threadIdx.x] = s;
-  }
+  reduce_intensity_body(partial + b * beam_stride, step_stride, n_tiles, rec, rec_len, sigma_off, n_steps, n_lines, dE,
+                        dvol, I_out + b * n_lines);
 }
 
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s) {

@@ -108,8 +108,9 @@ struct DiscArgs {
 // Launchers (defined in pnx_kernels.hip); return hipSuccess or the launch error.
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s);
-// Persistent forms over a device work list of (argument index, tile) items
-// with *n_items entries; `grid` blocks stride over the list.
+// Forms over a device work list of (argument index, tile) items with
+// *n_items entries: one block per item; `grid` is a multiple of 8 that is at
+// least *n_items.
 hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
                                     const int2* items, const int* n_items, int grid, hipStream_t s);
 hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,

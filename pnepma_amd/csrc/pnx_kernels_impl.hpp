@@ -36,6 +36,15 @@ namespace pnx {
 constexpr int kMaxLines = 8;
 constexpr int kMmsStride = 16;
 
+// Pointers into device memory (address space 1).  Accesses through them
+// compile to global_load / global_store; through a generic pointer read from
+// an argument block in memory they would be flat accesses, which also count
+// against the scalar-load counter and so serialise with the argument loads.
+using gcd = const __attribute__((address_space(1))) double*;
+using gd = __attribute__((address_space(1))) double*;
+__device__ __forceinline__ gcd glb(const double* p) { return (gcd)p; }
+__device__ __forceinline__ gd glb(double* p) { return (gd)p; }
+
 template <int N, int C>
 constexpr int ncls() {
   return Tab<N>::t.n[C];
@@ -56,10 +65,9 @@ __device__ __forceinline__ double expm1div(double c) {
 // du += sum over non-singleton axes of A_d[C] (u_F[R] - u_F[L]) / h_d, with
 // boundary ghosts of F rebuilt from this point's own values u (length n_C).
 template <int N, int DIM, int C, bool SRC>
-__device__ __forceinline__ void stencil(const Layout& L, const double* __restrict__ from,
-                                        const double* __restrict__ coef, const double* u, double* du, int i, int j,
-                                        int k, long p, double gfac, const double* __restrict__ prof_b,
-                                        const double* __restrict__ mms) {
+__device__ __forceinline__ void stencil(const Layout& L, gcd __restrict__ from, gcd __restrict__ coef,
+                                        const double* u, double* du, int i, int j, int k, long p, double gfac,
+                                        gcd __restrict__ prof_b, gcd __restrict__ mms) {
   constexpr int NC = ncls<N, C>();
   const long V = L.V;
   static_for<0, DIM>([&](auto Dc) {
@@ -71,7 +79,7 @@ __device__ __forceinline__ void stencil(const Layout& L, const double* __restric
       const long st = D == 0 ? 1L : (D == 1 ? (long)L.X : (long)L.X * L.Y);
       const long pL = codd ? p - st : p;
       const long pR = codd ? p : p + st;
-      const double* __restrict__ uf = from + L.cls_off[F];
+      gcd __restrict__ uf = from + L.cls_off[F];
       double vL[NF], vR[NF];
 #pragma unroll
       for (int J = 0; J < NF; ++J) {
@@ -84,7 +92,7 @@ __device__ __forceinline__ void stencil(const Layout& L, const double* __restric
         const bool left = idx == 0, right = idx == M - 1;
         if (left || right) {
           constexpr int boff = Tab<N>::t.b_off[F][D];
-          const double* __restrict__ Bm = coef + boff;
+          gcd __restrict__ Bm = coef + boff;
           double pr = 0.0;
           if constexpr (SRC && D == 0) pr = prof_b[F * (long)L.Y * L.Z + j + (long)L.Y * k];
 #pragma unroll
@@ -121,58 +129,75 @@ __device__ __forceinline__ void stencil(const Layout& L, const double* __restric
 
 // S and the per-degree C of a point (weighted_sum!, pn_problem.jl:213-220).
 template <int N>
-__device__ __forceinline__ void mix(const double* __restrict__ rc, long V, long p, int ne,
-                                    const double* __restrict__ rec, double& S, double* Cl) {
+__device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne, gcd __restrict__ rec, double& S,
+                                    double* Cl) {
   S = 0.0;
 #pragma unroll
   for (int l = 0; l <= N; ++l) Cl[l] = 0.0;
   for (int e = 0; e < ne; ++e) {
     const double r = rc[e * V + p];
     S += r * rec[e];
-    const double* __restrict__ ce = rec + ne + e * (N + 1);
+    gcd __restrict__ ce = rec + ne + e * (N + 1);
 #pragma unroll
     for (int l = 0; l <= N; ++l) Cl[l] += r * ce[l];
   }
 }
 
+// The per-step operands of one half step: the buffers it reads and writes, the
+// energy record and the step length.  Grid kernels take them from their
+// HalfStepArgs; the march kernel varies them step by step.
+struct StepView {
+  const double* from;  // other parity (beam 0)
+  const double* src;   // updated parity, read (beam 0)
+  double* dst;         // updated parity, written (beam 0)
+  const double* rec;   // energy record
+  double* partial;     // intensity partials of this step (forward even half step)
+  double dt;
+};
+
+__device__ __forceinline__ StepView view_of(const HalfStepArgs& a) {
+  return StepView{a.from, a.to_src, a.to_dst, a.rec, a.partial, a.dt};
+}
+
 template <int N, int DIM, int C, int MODE, int PAR>
-__device__ __forceinline__ void update_class(const HalfStepArgs& a, const double* __restrict__ from,
-                                             const double* src, double* dst, int i, int j, int k, long p, int beam,
-                                             double* contrib) {
+__device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+                                             gcd __restrict__ from, gcd src, gd dst, int i, int j, int k, long p,
+                                             int beam, double* contrib) {
   // src and dst are the same buffer for in-place steps: the own values are
   // read before they are overwritten, so neither pointer is restrict.
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
   const long V = L.V;
   const int ne = L.n_e;
-  const double* uc = src + L.cls_off[C];
+  gcd uc = src + L.cls_off[C];
   double u[NC], du[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
     u[m] = uc[m * V + p];
     du[m] = 0.0;
   }
-  const double gfac = a.rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, MODE == kForward>(L, from, a.coef, u, du, i, j, k, p, gfac, a.prof + beam * a.prof_stride,
-                                       a.mms);
+  const double gfac = rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, MODE == kForward>(L, from, glb(a.coef), u, du, i, j, k, p, gfac,
+                                       glb(a.prof) + beam * a.prof_stride, glb(a.mms));
   double S, Cl[N + 1];
-  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
+  mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
   double phi[N + 1];
   const double invS = 1.0 / S;
   static_for<0, N + 1>([&](auto lc) {
     constexpr int l = decltype(lc)::value;
-    if constexpr (class_has_degree<N, C>(l)) phi[l] = expm1div(-Cl[l] * invS * a.dt);
+    if constexpr (class_has_degree<N, C>(l)) phi[l] = expm1div(-Cl[l] * invS * dt);
   });
   double q0 = 0.0;
   if constexpr (MODE == kAdjoint && C == 0) {
     // compute_adjoint_source! (x_ray_intensities.jl:155-170), eee m = 0 only
     if (a.ibar != nullptr) {
-      const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
-      const double* __restrict__ ib = a.ibar + beam * L.n_lines;
-      for (int r = 0; r < L.n_lines; ++r) q0 -= sig[r] * a.W[r * V + p] * ib[r];
+      gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
+      gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+      gcd __restrict__ Wp = glb(a.W);
+      for (int r = 0; r < L.n_lines; ++r) q0 -= sig[r] * Wp[r * V + p] * ib[r];
     }
   }
-  double* out = dst + L.cls_off[C];
+  gd out = dst + L.cls_off[C];
   static_for<0, NC>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     constexpr int l = Tab<N>::t.l[C][I];
@@ -185,10 +210,10 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, const double
     }
     t = t * invS;
     t = t * phi[l];
-    const double un = u[I] + a.dt * t;
+    const double un = u[I] + dt * t;
     out[I * V + p] = un;
     if constexpr (MODE == kForward && PAR == 0 && C == 0 && I == 0) {
-      for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = a.W[r * V + p] * un;
+      for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = glb(a.W)[r * V + p] * un;
     }
   });
 }
@@ -197,6 +222,41 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// Intensities of one beam from the per-step, per-tile partial sums
+// (integrate_step_and_add!, x_ray_intensities.jl:172-191): thread t sums the
+// steps k = 1 + t, 1 + t + 256, ..., then the block reduces.  Every thread of
+// the block must call it.
+__device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int n_tiles,
+                                                      const double* __restrict__ rec, int rec_len, int sigma_off,
+                                                      int n_steps, int n_lines, double dE, double dvol,
+                                                      double* __restrict__ I_out) {
+  __shared__ double red[kBlock / 64][kMaxLines];
+  double acc[kMaxLines];
+#pragma unroll
+  for (int r = 0; r < kMaxLines; ++r) acc[r] = 0.0;
+  for (int kk = 1 + threadIdx.x; kk <= n_steps; kk += kBlock) {
+    gcd __restrict__ ps = glb(partial) + kk * step_stride;
+    gcd __restrict__ sig = glb(rec) + (long)kk * rec_len + sigma_off;
+    for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
+      double integral = 0.0;
+      for (int t = 0; t < n_tiles; ++t) integral += ps[(long)t * n_lines + r];
+      acc[r] += dE * dvol * sig[r] * integral * 1.0;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
+    const double s = wave_sum(acc[r]);
+    if (lane == 0) red[wave][r] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_lines && threadIdx.x < kMaxLines) {
+    double s = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    glb(I_out)[threadIdx.x] = s;
+  }
+  __syncthreads();
 }
 
 // Maps the linear block id to (beam, tile) so that each XCD (blocks are
@@ -210,8 +270,19 @@ __device__ __forceinline__ void block_coords(int n_tiles, int n_beams, int& beam
   tile = logical - beam * n_tiles;
 }
 
+// Work item of this block in a grouped kernel (one item per block; a
+// persistent loop over items makes the compiler keep far more registers live).
+// Blocks are dealt round-robin over the 8 XCDs; with a grid that is a
+// multiple of 8, XCD x takes the contiguous items x*G/8 .. (x+1)*G/8-1, so
+// the consecutive tiles of one request (and their shared neighbour rows) stay
+// in one XCD's L2.
+__device__ __forceinline__ int grouped_first() {
+  const int G = gridDim.x;
+  return (G & 7) == 0 ? (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+}
+
 template <int N, int DIM, int PAR, int MODE>
-__device__ __forceinline__ void half_step_body(const HalfStepArgs& a, int beam, int tile) {
+__device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const StepView& v, int beam, int tile) {
   const Layout& L = a.L;
   const int q = tile * kBlock + threadIdx.x;
   const bool valid = q < L.n_cells;
@@ -234,7 +305,8 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, int beam, 
         const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
                                 (DIM >= 3 && odd_in(C, 2) && k == 0));
         if (interior)
-          update_class<N, DIM, C, MODE, PAR>(a, a.from + bo, a.to_src + bo, a.to_dst + bo, i, j, k, p, beam, contrib);
+          update_class<N, DIM, C, MODE, PAR>(a, glb(v.rec), v.dt, glb(v.from) + bo, glb(v.src) + bo,
+                                             glb(v.dst) + bo, i, j, k, p, beam, contrib);
       }
     });
   }
@@ -250,7 +322,7 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, int beam, 
       double s = 0.0;
 #pragma unroll
       for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-      a.partial[beam * a.partial_stride + (long)tile * L.n_lines + threadIdx.x] = s;
+      glb(v.partial)[beam * a.partial_stride + (long)tile * L.n_lines + threadIdx.x] = s;
     }
     __syncthreads();
   }
@@ -260,20 +332,20 @@ template <int N, int DIM, int PAR, int MODE>
 __global__ __launch_bounds__(kBlock) void half_step_kernel(HalfStepArgs a) {
   int beam, tile;
   block_coords(a.L.n_tiles, a.n_beams, beam, tile);
-  half_step_body<N, DIM, PAR, MODE>(a, beam, tile);
+  half_step_body<N, DIM, PAR, MODE>(a, view_of(a), beam, tile);
 }
 
-// Persistent form for heterogeneous batches: blocks stride over a work list of
-// (argument block, tile) items, each argument block describing one request.
+// Form for heterogeneous batches: block -> one (argument block, tile) item of a
+// work list, each argument block describing one request.
 template <int N, int DIM, int PAR, int MODE>
 __global__ __launch_bounds__(kBlock) void half_step_grouped(const HalfStepArgs* __restrict__ args,
                                                             const int2* __restrict__ items,
                                                             const int* __restrict__ n_items) {
-  const int n = *n_items;
-  for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
-    half_step_body<N, DIM, PAR, MODE>(args[w.x], 0, w.y);
-  }
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const HalfStepArgs& a = args[w.x];
+  half_step_body<N, DIM, PAR, MODE>(a, view_of(a), 0, w.y);
 }
 
 // integrate_adjoint_densities! (pn_solver/pn_intensities.jl:129-187) for one
@@ -281,24 +353,26 @@ __global__ __launch_bounds__(kBlock) void half_step_grouped(const HalfStepArgs* 
 // rho_bar += lambda (s du + c u) dvol dE int_fac, plus the x-ray generation
 // pullback of integrate_step_and_add_pullback! at eee points.
 template <int N, int DIM, int C>
-__device__ __forceinline__ void dens_class(const DensArgs& a, const double* __restrict__ u_b,
-                                           const double* __restrict__ lam_b, int i, int j, int k, long p, int beam) {
+__device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ rec, double int_fac,
+                                           gcd __restrict__ u_b, gcd __restrict__ lam_b, int i, int j, int k, long p,
+                                           int beam) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
   const long V = L.V;
   const int ne = L.n_e;
-  const double* __restrict__ uc = u_b + L.cls_off[C];
+  gcd __restrict__ uc = u_b + L.cls_off[C];
   double u[NC], du[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
     u[m] = uc[m * V + p];
     du[m] = 0.0;
   }
-  const double gfac = a.rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, true>(L, u_b, a.coef, u, du, i, j, k, p, gfac, a.prof + beam * a.prof_stride, a.mms);
+  const double gfac = rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, true>(L, u_b, glb(a.coef), u, du, i, j, k, p, gfac, glb(a.prof) + beam * a.prof_stride,
+                          glb(a.mms));
   double S, Cl[N + 1];
-  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
-  const double* __restrict__ lc = lam_b + L.cls_off[C];
+  mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
+  gcd __restrict__ lc = lam_b + L.cls_off[C];
   double lam[NC], t[NC];
   static_for<0, NC>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
@@ -306,9 +380,9 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, const double* __re
     lam[I] = lc[I * V + p];
     t[I] = ((-du[I]) - Cl[l] * u[I]) / S;
   });
-  double* __restrict__ rb = a.rho_bar + beam * a.rb_stride + a.rb_off[C];
-  const double* __restrict__ s = a.rec;
-  const double* __restrict__ cl = a.rec + ne;
+  gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
+  gcd __restrict__ s = rec;
+  gcd __restrict__ cl = rec + ne;
   const double w = a.L.dvol;
   for (int e = 0; e < ne; ++e) {
     double acc = rb[e * V + p];
@@ -316,23 +390,24 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, const double* __re
     static_for<0, NC>([&](auto Ic) {
       constexpr int I = decltype(Ic)::value;
       constexpr int l = Tab<N>::t.l[C][I];
-      acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * a.int_fac;
+      acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
     });
     rb[e * V + p] = acc;
   }
   if constexpr (C == 0) {
-    const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
-    const double* __restrict__ ib = a.ibar + beam * L.n_lines;
-    double* __restrict__ g = a.gen + beam * a.gen_stride;
+    gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
+    gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+    gd __restrict__ g = glb(a.gen) + beam * a.gen_stride;
     for (int r = 0; r < L.n_lines; ++r) {
-      const double integral = a.dE * w * sig[r] * a.int_fac * ib[r];
+      const double integral = a.dE * w * sig[r] * int_fac * ib[r];
       g[r * V + p] += u[0] * integral;
     }
   }
 }
 
 template <int N, int DIM>
-__device__ __forceinline__ void dens_body(const DensArgs& a, int beam, int tile) {
+__device__ __forceinline__ void dens_body(const DensArgs& a, const double* u, const double* rec, double int_fac,
+                                          int beam, int tile) {
   const Layout& L = a.L;
   const int q = tile * kBlock + threadIdx.x;
   if (q >= L.n_cells) return;
@@ -347,7 +422,7 @@ __device__ __forceinline__ void dens_body(const DensArgs& a, int beam, int tile)
     if constexpr (class_active(C, DIM) && ncls<N, C>() > 0) {
       const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
                               (DIM >= 3 && odd_in(C, 2) && k == 0));
-      if (interior) dens_class<N, DIM, C>(a, a.u + bo, a.lam + bo, i, j, k, p, beam);
+      if (interior) dens_class<N, DIM, C>(a, glb(rec), int_fac, glb(u) + bo, glb(a.lam) + bo, i, j, k, p, beam);
     }
   });
 }
@@ -356,17 +431,17 @@ template <int N, int DIM>
 __global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
   int beam, tile;
   block_coords(a.L.n_tiles, a.n_beams, beam, tile);
-  dens_body<N, DIM>(a, beam, tile);
+  dens_body<N, DIM>(a, a.u, a.rec, a.int_fac, beam, tile);
 }
 
 template <int N, int DIM>
 __global__ __launch_bounds__(kBlock) void dens_grouped(const DensArgs* __restrict__ args, const int2* __restrict__ items,
                                                        const int* __restrict__ n_items) {
-  const int n = *n_items;
-  for (int it = blockIdx.x; it < n; it += gridDim.x) {
-    const int2 w = items[it];
-    dens_body<N, DIM>(args[w.x], 0, w.y);
-  }
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const DensArgs& a = args[w.x];
+  dens_body<N, DIM>(a, a.u, a.rec, a.int_fac, 0, w.y);
 }
 
 // ----------------------------------------------------------- exact adjoint
@@ -391,7 +466,7 @@ __device__ __forceinline__ double dexpm1div(double x) {
 
 // acc[NF] += sign/h * A_D[C]^T w  for one neighbour point of class C
 template <int N, int C, int D>
-__device__ __forceinline__ void at_w(const double* __restrict__ coef, const double* __restrict__ wc, long V, long q,
+__device__ __forceinline__ void at_w(gcd __restrict__ coef, gcd __restrict__ wc, long V, long q,
                                      double scale, double* acc) {
   constexpr int NC = ncls<N, C>();
   constexpr int F = switch_axis(C, D);
@@ -429,18 +504,18 @@ __device__ __forceinline__ void disc_b(const DiscArgs& a, int beam, int i, int j
       const int iq = D == 0 ? i : (D == 1 ? j : k);
       const int M = D == 0 ? L.Mx : (D == 1 ? L.My : L.Mz);
       const double ih = L.inv_h[D];
-      const double* __restrict__ wc = a.w_in + bo + L.cls_off[C];
+      gcd __restrict__ wc = glb(a.w_in) + bo + L.cls_off[C];
       if constexpr (odd_in(C, D)) {  // F even along D: read by C at iq (as R) and iq+1 (as L)
-        if (iq >= 1) at_w<N, C, D>(a.coef, wc, V, p, -ih, acc);
-        if (iq + 1 <= M - 1) at_w<N, C, D>(a.coef, wc, V, p + st, ih, acc);
+        if (iq >= 1) at_w<N, C, D>(glb(a.coef), wc, V, p, -ih, acc);
+        if (iq + 1 <= M - 1) at_w<N, C, D>(glb(a.coef), wc, V, p + st, ih, acc);
       } else {  // F odd along D (iq in 1..M-1): read by C at iq (as L) and iq-1 (as R); ghosts double
         const double cl = iq == M - 1 ? 2.0 : 1.0, cr = iq == 1 ? 2.0 : 1.0;
-        at_w<N, C, D>(a.coef, wc, V, p, cl * ih, acc);
-        at_w<N, C, D>(a.coef, wc, V, p - st, -cr * ih, acc);
+        at_w<N, C, D>(glb(a.coef), wc, V, p, cl * ih, acc);
+        at_w<N, C, D>(glb(a.coef), wc, V, p - st, -cr * ih, acc);
       }
     }
   });
-  double* __restrict__ lf = a.lam + bo + L.cls_off[F];
+  gd __restrict__ lf = glb(a.lam) + bo + L.cls_off[F];
 #pragma unroll
   for (int J = 0; J < NF; ++J) lf[J * V + p] += acc[J];
 }
@@ -453,18 +528,20 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
   const long V = L.V;
   const int ne = L.n_e;
   const long bo = (long)beam * L.beam_stride;
-  const double* __restrict__ uc = a.s_own + bo + L.cls_off[C];
+  gcd __restrict__ uc = glb(a.s_own) + bo + L.cls_off[C];
   double u[NC], d[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
     u[m] = uc[m * V + p];
     d[m] = 0.0;
   }
-  const double gfac = a.rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, true>(L, a.s_from + bo, a.coef, u, d, i, j, k, p, gfac, a.prof + beam * a.prof_stride, a.mms);
+  gcd __restrict__ rec = glb(a.rec);
+  const double gfac = rec[ne + ne * (N + 1)];
+  stencil<N, DIM, C, true>(L, glb(a.s_from) + bo, glb(a.coef), u, d, i, j, k, p, gfac,
+                          glb(a.prof) + beam * a.prof_stride, glb(a.mms));
   double S, Cl[N + 1];
-  mix<N>(a.rho + a.rho_off[C], V, p, ne, a.rec, S, Cl);
-  double* lc = a.lam + bo + L.cls_off[C];
+  mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
+  gd lc = glb(a.lam) + bo + L.cls_off[C];
   double lam[NC], w[NC], dS[NC], dC[NC];
   const double tau = a.tau;
   static_for<0, NC>([&](auto Ic) {
@@ -504,22 +581,22 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
         for (int I = 0; I < NC; ++I) {
           double bt = 0.0;
 #pragma unroll
-          for (int J = 0; J < NF; ++J) bt += a.coef[boff + J * NC + I] * at[J];
+          for (int J = 0; J < NF; ++J) bt += glb(a.coef)[boff + J * NC + I] * at[J];
           lam[I] -= s2 * bt;
         }
       }
     }
   });
-  double* __restrict__ wo = a.w_out + bo + L.cls_off[C];
+  gd __restrict__ wo = glb(a.w_out) + bo + L.cls_off[C];
 #pragma unroll
   for (int I = 0; I < NC; ++I) {
     lc[I * V + p] = lam[I];
     wo[I * V + p] = w[I];
   }
-  double* __restrict__ rb = a.rho_bar + beam * a.rb_stride + a.rho_off[C];
+  gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rho_off[C];
   for (int e = 0; e < ne; ++e) {
-    const double se = a.rec[e];
-    const double* __restrict__ ce = a.rec + ne + e * (N + 1);
+    const double se = rec[e];
+    gcd __restrict__ ce = rec + ne + e * (N + 1);
     double acc = 0.0;
     static_for<0, NC>([&](auto Ic) {
       constexpr int I = decltype(Ic)::value;
@@ -553,17 +630,17 @@ __device__ __forceinline__ void disc_body(const DiscArgs& a, int beam, int tile)
     const long V = L.V;
     const long bo = (long)beam * L.beam_stride;
     const int ne = L.n_e;
-    const double* __restrict__ sig = a.rec + ne + ne * (N + 1) + 1;
-    const double* __restrict__ ib = a.ibar + beam * L.n_lines;
-    const double u0 = a.u_direct[bo + L.cls_off[0] + p];
+    gcd __restrict__ sig = glb(a.rec) + ne + ne * (N + 1) + 1;
+    gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+    const double u0 = glb(a.u_direct)[bo + L.cls_off[0] + p];
     double add = 0.0;
-    double* __restrict__ g = a.gen + beam * a.gen_stride;
+    gd __restrict__ g = glb(a.gen) + beam * a.gen_stride;
     for (int rr = 0; rr < L.n_lines; ++rr) {
       const double wk = a.dE * L.dvol * sig[rr] * ib[rr];
-      add += wk * a.W[rr * V + p];
+      add += wk * glb(a.W)[rr * V + p];
       g[rr * V + p] += u0 * wk;
     }
-    a.lam[bo + L.cls_off[0] + p] += add;
+    glb(a.lam)[bo + L.cls_off[0] + p] += add;
   }
   static_for<0, 8>([&](auto Cc) {
     constexpr int C = decltype(Cc)::value;

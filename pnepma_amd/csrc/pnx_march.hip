@@ -1,0 +1,98 @@
+// March kernel: one workgroup advances one 1D request through a slice of its
+// forward+adjoint sequence (march_next), with the half steps, density
+// pullbacks, checkpoint copies and the intensity reduction separated by
+// workgroup barriers instead of kernel boundaries.  A 1D request has at most
+// a few hundred cells (one or two tiles of 256), so a workgroup holds all of
+// it and a step costs a few barrier round trips rather than seven launches.
+// The device functions are the ones the grid kernels use, in the same order
+// and with the same tiles, so the results are bitwise those of the grid path.
+#include <hip/hip_runtime.h>
+
+#include "pnx_kernels_impl.hpp"
+#include "pnx_march.hpp"
+
+namespace pnx {
+namespace {
+
+__device__ __forceinline__ void block_copy(double* dst_, const double* src_, long n) {
+  gd __restrict__ dst = glb(dst_);
+  gcd __restrict__ src = glb(src_);
+  for (long x = threadIdx.x; x < n; x += kBlock) dst[x] = src[x];
+}
+
+template <int N, int PAR, int MODE>
+__device__ __forceinline__ void half_pass(const HalfStepArgs& a, const StepView& v) {
+  for (int t = 0; t < a.L.n_tiles; ++t) half_step_body<N, 1, PAR, MODE>(a, v, 0, t);
+  __syncthreads();
+}
+
+template <int N>
+__device__ __forceinline__ void dens_pass(const DensArgs& a, const double* u, const double* rec, double int_fac) {
+  for (int t = 0; t < a.L.n_tiles; ++t) dens_body<N, 1>(a, u, rec, int_fac, 0, t);
+  __syncthreads();
+}
+
+__device__ __forceinline__ double* march_buffer(const MarchArgs& m, int buf, int idx, long st) {
+  if (buf == kBufCkpt) return m.ckpt + (long)idx * st;
+  if (buf == kBufSeg) return m.seg + (long)idx * st;
+  return m.state;
+}
+
+template <int N>
+__device__ __forceinline__ void march_body(const MarchArgs& m) {
+  const Layout& L = m.fwd.L;
+  const long st = L.beam_stride;
+  const double dE = m.dens.dE, h = dE / 2;
+  MarchPos pos = m.pos;
+  for (int op = 0; op < m.n_ops; ++op) {
+    const MarchOp o = march_next(pos, m.n, m.K, m.nck);
+    const double* src = march_buffer(m, o.src_buf, o.src_idx, st);
+    if (o.kind == kOpFwd || o.kind == kOpRecompute) {
+      if (o.ckpt_copy >= 0) {
+        block_copy(m.ckpt + (long)o.ckpt_copy * st, m.state, st);
+        __syncthreads();
+      }
+      double* dst = march_buffer(m, o.dst_buf, o.dst_idx, st);
+      const double* rec = m.rec_f + (long)o.step * m.rec_len;
+      // the recompute's intensity partials go to the scratch slot of step 0
+      double* part = o.kind == kOpFwd ? m.partial + (long)o.step * m.partial_step : m.partial;
+      half_pass<N, 1, kForward>(m.fwd, StepView{src, src, dst, rec, nullptr, h});
+      half_pass<N, 0, kForward>(m.fwd, StepView{dst, src, dst, rec, part, dE});
+      half_pass<N, 1, kForward>(m.fwd, StepView{dst, dst, dst, rec, nullptr, h});
+      if (o.reduce)
+        reduce_intensity_body(m.partial, m.partial_step, L.n_tiles, m.rec_f, m.rec_len, m.sigma_off, m.n, L.n_lines,
+                              dE, L.dvol, m.I_out);
+    } else if (o.kind == kOpReverse) {
+      dens_pass<N>(m.dens, src, m.rec_d + (long)o.step * m.rec_len, 1.0);
+      const double* rec = m.rec_a + (long)o.step * m.rec_len;
+      half_pass<N, 1, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, h});
+      half_pass<N, 0, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, dE});
+      half_pass<N, 1, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, h});
+    } else if (o.kind == kOpFinal) {
+      dens_pass<N>(m.dens, src, m.rec_d, 0.5);
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs* __restrict__ args, int n) {
+  for (int r = blockIdx.x; r < n; r += gridDim.x) march_body<N>(args[r]);
+}
+
+}  // namespace
+
+hipError_t launch_march(int N, const MarchArgs* args, int n_requests, hipStream_t s) {
+  if (n_requests <= 0) return hipSuccess;
+  const dim3 grid(n_requests < 8192 ? n_requests : 8192), block(kBlock);
+  switch (N) {
+    case 1: hipLaunchKernelGGL(march_kernel<1>, grid, block, 0, s, args, n_requests); break;
+    case 3: hipLaunchKernelGGL(march_kernel<3>, grid, block, 0, s, args, n_requests); break;
+    case 5: hipLaunchKernelGGL(march_kernel<5>, grid, block, 0, s, args, n_requests); break;
+    case 7: hipLaunchKernelGGL(march_kernel<7>, grid, block, 0, s, args, n_requests); break;
+    case 9: hipLaunchKernelGGL(march_kernel<9>, grid, block, 0, s, args, n_requests); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pnx

@@ -1,19 +1,24 @@
 // Continuous-batching server of forward+adjoint requests (pnx_server_* in include/pnx.h).
 //
-// A scheduler thread owns the device stream.  Each tick it
-//   1. admits arrived requests in arrival order while their device memory fits
-//      in the budget (one arena allocated at creation, first-fit sub-allocation;
-//      nothing is allocated per request on the device),
-//   2. advances every admitted request by one energy step of its current stage
-//      (forward step with checkpointing, forward recomputation of a segment, or
-//      reverse step: density pullback + adjoint step), following the same march
-//      as pnx_forward_adjoint,
-//   3. launches, for every (PN order, dimension) group and every one of the
-//      seven step slots, one persistent kernel over the work list of all
-//      requests in that slot (the grouped kernels), and
-//   4. completes requests whose results were copied back in an earlier tick.
-// Ticks are pipelined: the host prepares tick t while the device runs tick t-1;
-// per-tick argument blocks go through double-buffered pinned staging.
+// A scheduler thread admits arrived requests in arrival order while their
+// device memory fits in the budget (one arena allocated at creation,
+// first-fit sub-allocation; nothing is allocated per request on the device)
+// and hands each to a lane.  A lane owns one HIP stream and advances its
+// requests tick by tick, independently of the other lanes, so that the
+// kernels of different lanes overlap on the device:
+//   * 1D lane: one march launch per PN order per tick; a workgroup runs a
+//     slice of up to kMarchOps operations of one request (march_next), with
+//     workgroup barriers between the half steps;
+//   * 2D/3D lanes (split by PN order): one operation per request per tick; for
+//     every (PN order, dimension) group and every one of the seven step slots
+//     one kernel runs over the work list of all requests in that
+//     slot (the grouped kernels).
+// Every request follows the operation sequence of pnx_forward_adjoint with
+// one beam (march_next), with the same device functions, so its results are
+// bitwise those of the request solved alone.  A lane keeps at most two ticks
+// in flight: tick t's argument blocks go through pinned staging buffer t mod 2,
+// reused once tick t-2 has completed; requests complete once the tick that
+// copied their results back has completed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +35,7 @@
 #include <vector>
 
 #include "pnx_host.hpp"
+#include "pnx_march.hpp"
 
 using namespace pnx;
 
@@ -40,10 +46,13 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr int kSlots = 7;  // fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd
+constexpr int kSlots = 7;      // fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd
+constexpr int kMarchOps = 48;  // operations per 1D request per march launch
+constexpr int kLanes = 3;      // 1D; 2D/3D of orders <= 5; 2D/3D of orders 7, 9
+
+int lane_of(int N, int dims) { return dims == 1 ? 0 : (N <= 5 ? 1 : 2); }
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
-
 // First-fit sub-allocator over one device arena.
 class Arena {
  public:
@@ -101,8 +110,6 @@ class Arena {
   std::map<size_t, size_t> free_;
 };
 
-enum Stage { kFwd, kRecompute, kReverse, kFinal, kCopyOut, kDone };
-
 // doubles each device buffer of a request needs
 struct Sizes {
   long state, ckpt, seg, rho, W, rec, prof, partial, rb, gen, small;
@@ -132,25 +139,23 @@ struct Request {
   pnx_desc d{};
   Geometry g{};
   Sizes sz{};
+  int lane = 0;
   // host copies of the inputs (packed to the device layout at submit)
   std::vector<double> h_rec, h_rho, h_W, h_prof, h_mms, h_ibar;
   // outputs
   double* I_out = nullptr;
   double* rho_bar[8] = {};
   double* gen_out = nullptr;
-  std::vector<double> h_out;  // pinned copy-out lands here via staging
-  double* pinned_out = nullptr;
+  double* pinned_out = nullptr;  // results land here, then go to the caller's arrays
   // device
   size_t off = 0, bytes = 0;
   double *state = nullptr, *lam = nullptr, *ckpt = nullptr, *seg = nullptr, *rho = nullptr, *W = nullptr,
          *rec_f = nullptr, *rec_a = nullptr, *rec_d = nullptr, *prof = nullptr, *partial = nullptr, *rb = nullptr,
          *gen = nullptr, *mms = nullptr, *I_dev = nullptr, *ibar = nullptr;
   // schedule
-  Stage stage = kFwd;
-  int k = 1;       // next forward / recompute step
-  int i = 0;       // next reverse step
-  int segi = 0;    // current segment
-  int64_t done_tick = -1;
+  MarchPos pos = march_start();
+  int ops_left = 0;            // operations not yet issued
+  int64_t done_tick = -1;      // lane tick that copied the results back
   pnx_timing timing{};
   bool done = false;
 };
@@ -158,6 +163,20 @@ struct Request {
 struct GroupKey {
   int N, dims;
   bool operator<(const GroupKey& o) const { return N != o.N ? N < o.N : dims < o.dims; }
+};
+
+struct Lane {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  char* pinned[2] = {nullptr, nullptr};
+  size_t pinned_cap[2] = {0, 0};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int64_t ev_tick[2] = {-1, -1};  // tick whose completion ev[b] marks, -1 = none pending
+  char* dev_args = nullptr;
+  size_t dev_args_cap = 0;
+  int64_t tick = 0;          // next tick
+  int64_t done_through = -1; // every tick <= this one has completed on the device
+  std::vector<std::shared_ptr<Request>> active;
 };
 
 }  // namespace
@@ -172,20 +191,13 @@ struct pnx_server {
   std::condition_variable cv, done_cv;
   std::deque<std::shared_ptr<Request>> pending;
   std::map<int64_t, std::shared_ptr<Request>> all;
-  std::vector<std::shared_ptr<Request>> active;
+  int n_active = 0;
   int64_t next_ticket = 1;
   std::atomic<bool> stop{false};
   std::thread worker;
+  Lane lanes[kLanes];
   // coefficient buffers per order and direction
   std::map<int, double*> coef_f, coef_a;
-  // tick staging
-  std::vector<char> host_args;
-  char* pinned[2] = {nullptr, nullptr};
-  size_t pinned_cap[2] = {0, 0};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  bool ev_pending[2] = {false, false};
-  char* dev_args = nullptr;
-  size_t dev_args_cap = 0;
   int64_t ticks = 0, completed = 0, peak_active = 0;
   size_t peak_bytes = 0;
   int error = 0;
@@ -196,21 +208,36 @@ struct pnx_server {
 
 namespace {
 
-double* coef_for(pnx_server* s, int N, bool adjoint) {
-  auto& m = adjoint ? s->coef_a : s->coef_f;
-  auto it = m.find(N);
-  if (it != m.end()) return it->second;
-  std::vector<double> c = build_coefficients(N, adjoint);
-  double* p = nullptr;
-  if (hipMalloc(&p, c.size() * sizeof(double)) != hipSuccess) return nullptr;
-  if (hipMemcpy(p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  m[N] = p;
-  return p;
+constexpr int kOrders[] = {1, 3, 5, 7, 9};
+
+int upload_coefficients(pnx_server* s) {
+  for (int N : kOrders)
+    for (int adj = 0; adj < 2; ++adj) {
+      std::vector<double> c = build_coefficients(N, adj != 0);
+      double* p = nullptr;
+      if (hipMalloc(&p, c.size() * sizeof(double)) != hipSuccess) return -1;
+      (adj ? s->coef_a : s->coef_f)[N] = p;
+      if (hipMemcpy(p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    }
+  return 0;
 }
 
-// Carve the request's buffers out of its arena block and enqueue uploads + zero fills.
+const double* coef_for(pnx_server* s, int N, bool adjoint) { return (adjoint ? s->coef_a : s->coef_f).at(N); }
+
+int fail_lane(pnx_server* s, const char* what) {
+  (void)hipGetLastError();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!s->error) {
+    s->error = PNX_ERR_DEVICE;
+    s->error_msg = what;
+  }
+  return -1;
+}
+
+// Carve the request's buffers out of its arena block and enqueue uploads + zero
+// fills on its lane's stream.
 int admit(pnx_server* s, Request& r) {
-  hipStream_t st = s->ctx->stream;
+  hipStream_t st = s->lanes[r.lane].stream;
   const Sizes& z = r.sz;
   double* p = reinterpret_cast<double*>(s->arena.base() + r.off);
   auto take = [&](long n) {
@@ -235,29 +262,26 @@ int admit(pnx_server* s, Request& r) {
   r.I_dev = take(8);
   r.ibar = take(8);
   const long rec = z.rec;
-  if (hipMemcpyAsync(r.rec_f, r.h_rec.data(), rec * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.rec_a, r.h_rec.data() + rec, rec * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.rec_d, r.h_rec.data() + 2 * rec, rec * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.rho, r.h_rho.data(), r.h_rho.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.W, r.h_W.data(), r.h_W.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.prof, r.h_prof.data(), r.h_prof.size() * sizeof(double), hipMemcpyHostToDevice, st) !=
-          hipSuccess ||
-      hipMemcpyAsync(r.mms, r.h_mms.data(), r.h_mms.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(r.ibar, r.h_ibar.data(), r.h_ibar.size() * sizeof(double), hipMemcpyHostToDevice, st) !=
-          hipSuccess)
+  auto up = [&](double* dst, const double* src, size_t n) {
+    return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  if (!up(r.rec_f, r.h_rec.data(), rec) || !up(r.rec_a, r.h_rec.data() + rec, rec) ||
+      !up(r.rec_d, r.h_rec.data() + 2 * rec, rec) || !up(r.rho, r.h_rho.data(), r.h_rho.size()) ||
+      !up(r.W, r.h_W.data(), r.h_W.size()) || !up(r.prof, r.h_prof.data(), r.h_prof.size()) ||
+      !up(r.mms, r.h_mms.data(), r.h_mms.size()) || !up(r.ibar, r.h_ibar.data(), r.h_ibar.size()))
     return -1;
+  // the recompute writes whole segment slots; ghost slots are never read for
+  // values, but keep them finite
   if (launch_fill(r.state, z.state, 0.0, st) != hipSuccess || launch_fill(r.lam, z.state, 0.0, st) != hipSuccess ||
-      launch_fill(r.rb, z.rb, 0.0, st) != hipSuccess || launch_fill(r.gen, z.gen, 0.0, st) != hipSuccess)
+      launch_fill(r.rb, z.rb, 0.0, st) != hipSuccess || launch_fill(r.gen, z.gen, 0.0, st) != hipSuccess ||
+      launch_fill(r.seg, z.seg, 0.0, st) != hipSuccess)
     return -1;
-  // the recompute writes whole segment slots; ghost slots are never read for values,
-  // but keep them finite
-  if (launch_fill(r.seg, z.seg, 0.0, st) != hipSuccess) return -1;
-  r.stage = kFwd;
-  r.k = 1;
+  r.pos = march_start();
+  r.ops_left = march_ops(r.d.n_steps);
   return 0;
 }
 
-HalfStepArgs make_half(pnx_server* s, Request& r, int mode, const double* rec, double dt, const double* from,
+HalfStepArgs make_half(pnx_server* s, const Request& r, int mode, const double* rec, double dt, const double* from,
                        const double* src, double* dst, double* partial) {
   HalfStepArgs a{};
   a.L = r.g.L;
@@ -280,7 +304,7 @@ HalfStepArgs make_half(pnx_server* s, Request& r, int mode, const double* rec, d
   return a;
 }
 
-DensArgs make_dens(pnx_server* s, Request& r, int i, const double* u, double int_fac) {
+DensArgs make_dens(pnx_server* s, const Request& r, int i, const double* u, double int_fac) {
   DensArgs a{};
   a.L = r.g.L;
   a.u = u;
@@ -307,6 +331,37 @@ DensArgs make_dens(pnx_server* s, Request& r, int i, const double* u, double int
   return a;
 }
 
+MarchArgs make_march(pnx_server* s, const Request& r) {
+  const Geometry& g = r.g;
+  MarchArgs m{};
+  m.fwd = make_half(s, r, kForward, r.rec_f, g.dE, r.state, r.state, r.state, r.partial);
+  m.adj = make_half(s, r, kAdjoint, r.rec_a, g.dE, r.lam, r.lam, r.lam, nullptr);
+  m.dens = make_dens(s, r, 0, r.ckpt, 1.0);
+  m.state = r.state;
+  m.lam = r.lam;
+  m.ckpt = r.ckpt;
+  m.seg = r.seg;
+  m.partial = r.partial;
+  m.I_out = r.I_dev;
+  m.rec_f = r.rec_f;
+  m.rec_a = r.rec_a;
+  m.rec_d = r.rec_d;
+  m.partial_step = g.partial_step;
+  m.rec_len = g.rec_len;
+  m.sigma_off = g.sigma_off;
+  m.n = r.d.n_steps;
+  m.K = g.K;
+  m.nck = g.nck;
+  return m;
+}
+
+double* buffer_of(const Request& r, int buf, int idx) {
+  const long st = r.g.L.beam_stride;
+  if (buf == kBufCkpt) return r.ckpt + (long)idx * st;
+  if (buf == kBufSeg) return r.seg + (long)idx * st;
+  return r.state;
+}
+
 struct SlotWork {
   std::vector<HalfStepArgs> half;
   std::vector<DensArgs> dens;
@@ -314,10 +369,11 @@ struct SlotWork {
 };
 
 struct TickPlan {
-  std::map<GroupKey, SlotWork[kSlots]> groups;
-  std::vector<std::pair<Request*, int>> copies;    // (request, checkpoint index) copies before kernels
-  std::vector<Request*> reduce;                    // forward finished: reduce intensities after the slots
-  std::vector<Request*> copy_out;                  // results to copy back after the slots
+  std::map<GroupKey, SlotWork[kSlots]> groups;      // grid lanes
+  std::map<int, std::vector<MarchArgs>> march;      // 1D lane, per order
+  std::vector<std::pair<Request*, int>> copies;     // (request, checkpoint index) copies before the slots
+  std::vector<Request*> reduce;                     // forward finished: reduce intensities after slot 2
+  std::vector<Request*> copy_out;                   // results to copy back at the end of the tick
 };
 
 void add_half(SlotWork& w, const HalfStepArgs& a) {
@@ -331,71 +387,39 @@ void add_dens(SlotWork& w, const DensArgs& a) {
   for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
 }
 
-// Plan one energy step (of the current stage) for request r; advance its stage.
-void plan_request(pnx_server* s, Request& r, TickPlan& plan) {
+// Plan the next operation of request r on a grid lane.
+void plan_grid(pnx_server* s, Request& r, TickPlan& plan) {
   const Geometry& g = r.g;
-  const long st = g.L.beam_stride;
-  const int n = r.d.n_steps, K = g.K;
   SlotWork* w = plan.groups[GroupKey{r.d.N, g.L.dims}];
   const double h = g.dE / 2;
-  auto fwd3 = [&](int k, const double* src, double* dst, double* partial) {
-    const double* rec = r.rec_f + (long)k * g.rec_len;
-    add_half(w[0], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
-    add_half(w[1], make_half(s, r, kForward, rec, g.dE, dst, src, dst, partial));
-    add_half(w[2], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
-  };
-  switch (r.stage) {
-    case kFwd: {
-      const int k = r.k;
-      if ((k - 1) % K == 0) plan.copies.push_back({&r, (k - 1) / K});
-      fwd3(k, r.state, r.state, r.partial + (long)k * g.partial_step);
-      if (k == n) {
-        plan.reduce.push_back(&r);
-        r.stage = kRecompute;
-        r.segi = g.nck - 1;
-        r.k = r.segi * K + 1;
-      } else {
-        r.k = k + 1;
-      }
+  const MarchOp o = march_next(r.pos, r.d.n_steps, g.K, g.nck);
+  r.ops_left -= 1;
+  const double* src = buffer_of(r, o.src_buf, o.src_idx);
+  switch (o.kind) {
+    case kOpFwd:
+    case kOpRecompute: {
+      if (o.ckpt_copy >= 0) plan.copies.push_back({&r, o.ckpt_copy});
+      double* dst = buffer_of(r, o.dst_buf, o.dst_idx);
+      const double* rec = r.rec_f + (long)o.step * g.rec_len;
+      // the recompute's intensity partials go to the scratch slot of step 0
+      double* part = o.kind == kOpFwd ? r.partial + (long)o.step * g.partial_step : r.partial;
+      add_half(w[0], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
+      add_half(w[1], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part));
+      add_half(w[2], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
+      if (o.reduce) plan.reduce.push_back(&r);
       break;
     }
-    case kRecompute: {
-      const int k0 = r.segi * K, k1 = std::min(n, k0 + K), k = r.k;
-      const double* src = k == k0 + 1 ? r.ckpt + (long)r.segi * st : r.seg + (long)(k - k0 - 2) * st;
-      double* dst = r.seg + (long)(k - k0 - 1) * st;
-      fwd3(k, src, dst, r.partial);  // step-0 partial slot is scratch
-      if (k == k1) {
-        r.stage = kReverse;
-        r.i = k1;
-      } else {
-        r.k = k + 1;
-      }
-      break;
-    }
-    case kReverse: {
-      const int k0 = r.segi * K, i = r.i;
-      add_dens(w[3], make_dens(s, r, i, r.seg + (long)(i - k0 - 1) * st, 1.0));
-      const double* rec = r.rec_a + (long)i * g.rec_len;
+    case kOpReverse: {
+      add_dens(w[3], make_dens(s, r, o.step, src, 1.0));
+      const double* rec = r.rec_a + (long)o.step * g.rec_len;
       add_half(w[4], make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr));
       add_half(w[5], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr));
       add_half(w[6], make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr));
-      if (i == k0 + 1) {
-        if (r.segi > 0) {
-          r.segi -= 1;
-          r.stage = kRecompute;
-          r.k = r.segi * K + 1;
-        } else {
-          r.stage = kFinal;
-        }
-      } else {
-        r.i = i - 1;
-      }
       break;
     }
-    case kFinal:
-      add_dens(w[3], make_dens(s, r, 0, r.ckpt, 0.5));
+    case kOpFinal:
+      add_dens(w[3], make_dens(s, r, 0, src, 0.5));
       plan.copy_out.push_back(&r);
-      r.stage = kCopyOut;
       break;
     default:
       break;
@@ -403,40 +427,60 @@ void plan_request(pnx_server* s, Request& r, TickPlan& plan) {
   r.timing.ticks += 1;
 }
 
-int ensure_pinned(pnx_server* s, int b, size_t bytes) {
-  if (s->pinned_cap[b] >= bytes) return 0;
-  if (s->pinned[b]) (void)hipHostFree(s->pinned[b]);
-  s->pinned[b] = nullptr;
-  size_t cap = std::max(bytes, (size_t)1 << 20) * 2;
-  if (hipHostMalloc(&s->pinned[b], cap, hipHostMallocDefault) != hipSuccess) return -1;
-  s->pinned_cap[b] = cap;
+// Plan the next slice of operations of request r on the 1D lane.
+void plan_march(pnx_server* s, Request& r, TickPlan& plan) {
+  MarchArgs m = make_march(s, r);
+  m.pos = r.pos;
+  m.n_ops = std::min(kMarchOps, r.ops_left);
+  for (int i = 0; i < m.n_ops; ++i) (void)march_next(r.pos, r.d.n_steps, r.g.K, r.g.nck);
+  r.ops_left -= m.n_ops;
+  plan.march[r.d.N].push_back(m);
+  if (r.ops_left == 0) plan.copy_out.push_back(&r);
+  r.timing.ticks += 1;
+}
+
+int ensure_pinned(Lane& ln, int b, size_t bytes) {
+  if (ln.pinned_cap[b] >= bytes) return 0;
+  if (ln.pinned[b]) (void)hipHostFree(ln.pinned[b]);
+  ln.pinned[b] = nullptr;
+  ln.pinned_cap[b] = 0;
+  const size_t cap = std::max(bytes, (size_t)1 << 20) * 2;
+  if (hipHostMalloc(&ln.pinned[b], cap, hipHostMallocDefault) != hipSuccess) return -1;
+  ln.pinned_cap[b] = cap;
   return 0;
 }
 
-int ensure_dev_args(pnx_server* s, size_t bytes) {
-  if (s->dev_args_cap >= bytes) return 0;
-  // the stream must be idle before the old buffer goes away
-  if (hipStreamSynchronize(s->ctx->stream) != hipSuccess) return -1;
-  if (s->dev_args) (void)hipFree(s->dev_args);
-  s->dev_args = nullptr;
-  size_t cap = std::max(bytes, (size_t)1 << 20) * 2;
-  if (hipMalloc(&s->dev_args, cap) != hipSuccess) return -1;
-  s->dev_args_cap = cap;
+int ensure_dev_args(Lane& ln, size_t bytes) {
+  if (ln.dev_args_cap >= bytes) return 0;
+  // the lane's earlier ticks read the old buffer
+  if (hipStreamSynchronize(ln.stream) != hipSuccess) return -1;
+  if (ln.dev_args) (void)hipFree(ln.dev_args);
+  ln.dev_args = nullptr;
+  ln.dev_args_cap = 0;
+  const size_t cap = std::max(bytes, (size_t)1 << 20) * 2;
+  if (hipMalloc(&ln.dev_args, cap) != hipSuccess) return -1;
+  ln.dev_args_cap = cap;
   return 0;
 }
 
-// Serialise the plan's argument blocks and work lists, upload, and launch.
-int launch_plan(pnx_server* s, TickPlan& plan, int buf) {
-  hipStream_t st = s->ctx->stream;
-  // layout: per group, per slot: [n_items int, pad][args][items]
+// Serialise the plan's argument blocks and work lists into staging buffer
+// `buf`, upload them, and enqueue the tick on the lane's stream.
+int launch_plan(Lane& ln, TickPlan& plan, int buf) {
+  hipStream_t st = ln.stream;
   struct Entry {
     GroupKey key;
-    int slot;
+    int slot;  // -1: march block of order key.N
     size_t n_off, args_off, items_off;
     int n_items;
   };
   std::vector<Entry> entries;
   size_t off = 0;
+  for (auto& kv : plan.march) {
+    Entry e{GroupKey{kv.first, 1}, -1, 0, 0, 0, (int)kv.second.size()};
+    e.args_off = off;
+    off = align_up(off + kv.second.size() * sizeof(MarchArgs));
+    entries.push_back(e);
+  }
   for (auto& kv : plan.groups)
     for (int sl = 0; sl < kSlots; ++sl) {
       SlotWork& w = kv.second[sl];
@@ -451,9 +495,14 @@ int launch_plan(pnx_server* s, TickPlan& plan, int buf) {
       entries.push_back(e);
     }
   if (off == 0) off = kAlign;
-  if (ensure_pinned(s, buf, off) || ensure_dev_args(s, off)) return -1;
-  char* h = s->pinned[buf];
+  if (ensure_pinned(ln, buf, off) || ensure_dev_args(ln, off)) return -1;
+  char* h = ln.pinned[buf];
   for (const Entry& e : entries) {
+    if (e.slot < 0) {
+      const auto& v = plan.march[e.key.N];
+      std::memcpy(h + e.args_off, v.data(), v.size() * sizeof(MarchArgs));
+      continue;
+    }
     SlotWork& w = plan.groups[e.key][e.slot];
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
     if (e.slot == 3)
@@ -462,7 +511,7 @@ int launch_plan(pnx_server* s, TickPlan& plan, int buf) {
       std::memcpy(h + e.args_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
     std::memcpy(h + e.items_off, w.items.data(), w.items.size() * sizeof(int2));
   }
-  if (hipMemcpyAsync(s->dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+  if (hipMemcpyAsync(ln.dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
   for (auto& cp : plan.copies) {
     Request& r = *cp.first;
     const long stn = r.g.L.beam_stride;
@@ -470,21 +519,26 @@ int launch_plan(pnx_server* s, TickPlan& plan, int buf) {
         hipSuccess)
       return -1;
   }
+  for (const Entry& e : entries)
+    if (e.slot < 0 &&
+        launch_march(e.key.N, reinterpret_cast<const MarchArgs*>(ln.dev_args + e.args_off), e.n_items, st) !=
+            hipSuccess)
+      return -1;
   auto launch_slot = [&](int slot) -> int {
     for (const Entry& e : entries) {
       if (e.slot != slot) continue;
-      const int grid = std::min(e.n_items, 2048);
-      const int* nptr = reinterpret_cast<const int*>(s->dev_args + e.n_off);
-      const int2* items = reinterpret_cast<const int2*>(s->dev_args + e.items_off);
+      const int grid = (e.n_items + 7) & ~7;  // one block per item, a multiple of 8 for the XCD mapping
+      const int* nptr = reinterpret_cast<const int*>(ln.dev_args + e.n_off);
+      const int2* items = reinterpret_cast<const int2*>(ln.dev_args + e.items_off);
       hipError_t err;
       if (slot == 3) {
-        err = launch_dens_grouped(e.key.N, e.key.dims, reinterpret_cast<const DensArgs*>(s->dev_args + e.args_off),
+        err = launch_dens_grouped(e.key.N, e.key.dims, reinterpret_cast<const DensArgs*>(ln.dev_args + e.args_off),
                                   items, nptr, grid, st);
       } else {
         const int mode = slot < 3 ? kForward : kAdjoint;
         const int parity = (slot == 1 || slot == 5) ? 0 : 1;
         err = launch_half_step_grouped(e.key.N, e.key.dims, parity, mode,
-                                       reinterpret_cast<const HalfStepArgs*>(s->dev_args + e.args_off), items, nptr,
+                                       reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
                                        grid, st);
       }
       if (err != hipSuccess) return -1;
@@ -503,14 +557,13 @@ int launch_plan(pnx_server* s, TickPlan& plan, int buf) {
   for (int sl = 3; sl < kSlots; ++sl)
     if (launch_slot(sl)) return -1;
   for (Request* r : plan.copy_out) {
-    // results -> pinned staging of the request
     const long nI = r->d.n_lines, nrb = r->sz.rb, ngen = r->sz.gen;
     if (hipMemcpyAsync(r->pinned_out, r->I_dev, nI * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(r->pinned_out + nI, r->rb, nrb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(r->pinned_out + nI + nrb, r->gen, ngen * sizeof(double), hipMemcpyDeviceToHost, st) !=
             hipSuccess)
       return -1;
-    r->done_tick = s->ticks;
+    r->done_tick = ln.tick;
   }
   return 0;
 }
@@ -534,26 +587,62 @@ void finish_request(pnx_server* s, Request& r) {
   s->arena.release(r.off, r.bytes);
   r.timing.done_s = s->now();
   r.timing.status = PNX_OK;
-  r.stage = kDone;
   r.done = true;
-  r.h_rec.clear();
-  r.h_rec.shrink_to_fit();
-  r.h_rho.clear();
-  r.h_rho.shrink_to_fit();
-  r.h_W.clear();
-  r.h_W.shrink_to_fit();
+  std::vector<double>().swap(r.h_rec);
+  std::vector<double>().swap(r.h_rho);
+  std::vector<double>().swap(r.h_W);
+  std::vector<double>().swap(r.h_prof);
+}
+
+// Records which of the lane's in-flight ticks have completed.
+int poll_lane(Lane& ln) {
+  for (int b = 0; b < 2; ++b) {
+    if (ln.ev_tick[b] < 0) continue;
+    const hipError_t e = hipEventQuery(ln.ev[b]);
+    if (e == hipSuccess) {
+      ln.done_through = std::max(ln.done_through, ln.ev_tick[b]);
+      ln.ev_tick[b] = -1;
+    } else if (e != hipErrorNotReady) {
+      return -1;
+    }
+  }
+  return 0;
+}
+
+// Plans and enqueues the lane's next tick if its staging buffer is free.
+// Returns 1 if a tick was launched, 0 if not, -1 on error.
+int step_lane(pnx_server* s, int li) {
+  Lane& ln = s->lanes[li];
+  const int buf = (int)(ln.tick & 1);
+  if (ln.ev_tick[buf] >= 0) return 0;  // tick - 2 still in flight
+  TickPlan plan;
+  bool any = false;
+  for (auto& r : ln.active) {
+    if (r->ops_left == 0) continue;
+    any = true;
+    if (li == 0)
+      plan_march(s, *r, plan);
+    else
+      plan_grid(s, *r, plan);
+  }
+  if (!any) return 0;
+  if (launch_plan(ln, plan, buf)) return -1;
+  if (hipEventRecord(ln.ev[buf], ln.stream) != hipSuccess) return -1;
+  ln.ev_tick[buf] = ln.tick;
+  ln.tick += 1;
+  return 1;
 }
 
 void worker_loop(pnx_server* s) {
   (void)hipSetDevice(s->ctx->device);
-  int64_t tick = 0;
   while (true) {
     std::vector<std::shared_ptr<Request>> admitted;
     {
       std::unique_lock<std::mutex> lk(s->mu);
+      if (s->error) break;
       const double now = s->now();
       // admission in arrival order while memory and slots allow
-      while (!s->pending.empty() && (int)s->active.size() < s->max_active) {
+      while (!s->pending.empty() && s->n_active < s->max_active) {
         auto& r = s->pending.front();
         if (r->timing.arrival_s > now) break;
         const size_t off = s->arena.alloc(r->bytes);
@@ -561,10 +650,10 @@ void worker_loop(pnx_server* s) {
         r->off = off;
         r->timing.admit_s = now;
         admitted.push_back(r);
-        s->active.push_back(r);
+        s->n_active += 1;
         s->pending.pop_front();
       }
-      if (s->active.empty()) {
+      if (s->n_active == 0) {
         if (s->stop && s->pending.empty()) break;
         if (s->pending.empty()) {
           s->cv.wait_for(lk, std::chrono::milliseconds(5));
@@ -574,53 +663,58 @@ void worker_loop(pnx_server* s) {
         }
         continue;
       }
-      s->peak_active = std::max<int64_t>(s->peak_active, (int64_t)s->active.size());
+      s->peak_active = std::max<int64_t>(s->peak_active, (int64_t)s->n_active);
       s->peak_bytes = std::max(s->peak_bytes, s->arena.used());
     }
     for (auto& r : admitted) {
       if (admit(s, *r)) {
-        s->error = PNX_ERR_DEVICE;
-        s->error_msg = "request upload failed";
+        fail_lane(s, "request upload failed");
+        break;
       }
+      s->lanes[r->lane].active.push_back(r);
     }
-    const int buf = (int)(tick & 1);
-    // staging buffer `buf` was last used two ticks ago: its copy must be done
-    if (s->ev_pending[buf]) {
-      (void)hipEventSynchronize(s->ev[buf]);
-      s->ev_pending[buf] = false;
+    bool launched = false;
+    std::vector<std::shared_ptr<Request>> finished;
+    for (int li = 0; li < kLanes; ++li) {
+      Lane& ln = s->lanes[li];
+      if (ln.active.empty()) continue;
+      if (poll_lane(ln)) {
+        fail_lane(s, "device error in a served tick");
+        break;
+      }
+      // requests whose copy-out tick has completed
+      auto it = std::partition(ln.active.begin(), ln.active.end(), [&](const std::shared_ptr<Request>& r) {
+        return !(r->done_tick >= 0 && r->done_tick <= ln.done_through);
+      });
+      finished.insert(finished.end(), it, ln.active.end());
+      ln.active.erase(it, ln.active.end());
+      const int rc = step_lane(s, li);
+      if (rc < 0) {
+        fail_lane(s, "tick launch failed");
+        break;
+      }
+      launched |= rc > 0;
     }
-    TickPlan plan;
-    for (auto& r : s->active)
-      if (r->stage != kCopyOut) plan_request(s, *r, plan);
-    if (launch_plan(s, plan, buf)) {
-      s->error = PNX_ERR_DEVICE;
-      s->error_msg = "tick launch failed";
-    }
-    (void)hipEventRecord(s->ev[buf], s->ctx->stream);
-    s->ev_pending[buf] = true;
-    // complete requests whose copy-out was enqueued in an earlier tick that has finished
-    // (tick - 2 finished before this tick's staging was reused; tick - 1 is queried)
-    const int prev = 1 - buf;
-    const bool prev_done = !s->ev_pending[prev] || hipEventQuery(s->ev[prev]) == hipSuccess;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      std::vector<std::shared_ptr<Request>> keep;
-      for (auto& r : s->active) {
-        const bool fin = r->stage == kCopyOut && r->done_tick >= 0 &&
-                         (r->done_tick <= tick - 2 || (r->done_tick == tick - 1 && prev_done));
-        if (fin) {
+    if (!finished.empty()) {
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (auto& r : finished) {
           finish_request(s, *r);
           s->completed += 1;
-        } else {
-          keep.push_back(r);
+          s->n_active -= 1;
         }
       }
-      s->active.swap(keep);
-      s->ticks = ++tick;
+      s->done_cv.notify_all();
     }
-    s->done_cv.notify_all();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->ticks = s->lanes[0].tick + s->lanes[1].tick + s->lanes[2].tick;
+    }
+    if (!launched && finished.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
-  (void)hipStreamSynchronize(s->ctx->stream);
+  for (Lane& ln : s->lanes)
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+  s->done_cv.notify_all();
 }
 
 }  // namespace
@@ -639,6 +733,22 @@ int64_t pnx_request_device_bytes(const pnx_desc* desc) {
   return (int64_t)(z.total() * sizeof(double) + 20 * kAlign);
 }
 
+static void release_server(pnx_server* s) {
+  for (auto& kv : s->coef_f) (void)hipFree(kv.second);
+  for (auto& kv : s->coef_a) (void)hipFree(kv.second);
+  for (Lane& ln : s->lanes) {
+    for (int b = 0; b < 2; ++b) {
+      if (ln.pinned[b]) (void)hipHostFree(ln.pinned[b]);
+      if (ln.ev[b]) (void)hipEventDestroy(ln.ev[b]);
+    }
+    if (ln.dev_args) (void)hipFree(ln.dev_args);
+    if (ln.stream && ln.own_stream) (void)hipStreamDestroy(ln.stream);
+  }
+  for (auto& kv : s->all)
+    if (kv.second->pinned_out) (void)hipHostFree(kv.second->pinned_out);
+  delete s;
+}
+
 int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_active, pnx_server** out) {
   if (!ctx || !out || hbm_budget_bytes <= 0 || max_active < 1) return set_error(PNX_ERR_ARG, "bad argument");
   if (hipSetDevice(ctx->device) != hipSuccess) return set_error(PNX_ERR_DEVICE, "hipSetDevice failed");
@@ -647,14 +757,34 @@ int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_ac
   s->budget = hbm_budget_bytes;
   s->max_active = max_active;
   if (s->arena.init((size_t)hbm_budget_bytes)) {
-    delete s;
+    release_server(s);
     return set_error(PNX_ERR_MEMORY, "could not allocate the server arena");
   }
-  for (int b = 0; b < 2; ++b)
-    if (hipEventCreateWithFlags(&s->ev[b], hipEventDisableTiming) != hipSuccess) {
-      delete s;
-      return set_error(PNX_ERR_DEVICE, "hipEventCreate failed");
+  if (upload_coefficients(s)) {
+    release_server(s);
+    return set_error(PNX_ERR_DEVICE, "coefficient upload failed");
+  }
+  // The 1D lane runs on the context's stream: with the runtime's default of
+  // four hardware queues per process, two more streams leave every lane a
+  // queue of its own next to the default stream.
+  for (int li = 0; li < kLanes; ++li) {
+    Lane& ln = s->lanes[li];
+    if (li == 0) {
+      ln.stream = ctx->stream;
+    } else if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) != hipSuccess) {
+      ln.stream = nullptr;
+      release_server(s);
+      return set_error(PNX_ERR_DEVICE, "hipStreamCreate failed");
+    } else {
+      ln.own_stream = true;
     }
+    for (int b = 0; b < 2; ++b)
+      if (hipEventCreateWithFlags(&ln.ev[b], hipEventDisableTiming) != hipSuccess) {
+        ln.ev[b] = nullptr;
+        release_server(s);
+        return set_error(PNX_ERR_DEVICE, "hipEventCreate failed");
+      }
+  }
   s->t0 = std::chrono::steady_clock::now();
   s->worker = std::thread(worker_loop, s);
   *out = s;
@@ -670,17 +800,7 @@ int pnx_server_destroy(pnx_server* s) {
   s->cv.notify_all();
   if (s->worker.joinable()) s->worker.join();
   (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
-  for (auto& kv : s->coef_f) (void)hipFree(kv.second);
-  for (auto& kv : s->coef_a) (void)hipFree(kv.second);
-  for (int b = 0; b < 2; ++b) {
-    if (s->pinned[b]) (void)hipHostFree(s->pinned[b]);
-    if (s->ev[b]) (void)hipEventDestroy(s->ev[b]);
-  }
-  if (s->dev_args) (void)hipFree(s->dev_args);
-  for (auto& kv : s->all)
-    if (kv.second->pinned_out) (void)hipHostFree(kv.second->pinned_out);
-  delete s;
+  release_server(s);
   return PNX_OK;
 }
 
@@ -698,6 +818,7 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   int rc = make_geometry(r->d, r->g, err);
   if (rc) return set_error(rc, err);
   r->sz = request_sizes(r->g, r->d);
+  r->lane = lane_of(r->d.N, r->g.L.dims);
   r->bytes = (size_t)r->sz.total() * sizeof(double) + 20 * kAlign;
   if ((int64_t)r->bytes > s->budget) return set_error(PNX_ERR_MEMORY, "request exceeds the server's HBM budget");
   const Geometry& g = r->g;
@@ -772,7 +893,7 @@ int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t) {
 int pnx_server_drain(pnx_server* s) {
   if (!s) return set_error(PNX_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(s->mu);
-  s->done_cv.wait(lk, [&] { return (s->pending.empty() && s->active.empty()) || s->error != 0; });
+  s->done_cv.wait(lk, [&] { return (s->pending.empty() && s->n_active == 0) || s->error != 0; });
   if (s->error) return set_error(s->error, s->error_msg);
   return PNX_OK;
 }
