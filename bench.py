@@ -97,6 +97,7 @@ def serve_run(preps, rate, budget, max_active):
     srv = serving.Server(budget, max_active=max_active)
     t0 = srv.clock() + 0.05
     tickets = [srv.submit(pr, arrival_s=t0 + a) for pr, a in zip(preps, arr)]
+    progress(f"submitted {len(tickets)} requests in {srv.clock() - t0 + 0.05:.3f} s")
     srv.drain()
     tm = [srv.wait(t) for t in tickets]
     stats = srv.stats()

@@ -126,6 +126,20 @@ __global__ __launch_bounds__(kBlock) void reduce_intensity_kernel(const double* 
                         dvol, I_out + b * n_lines);
 }
 
+__global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long x = i; x < n; x += stride) dst[x] = src[x];
+}
+
+hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   long blocks = (n + kBlock - 1) / kBlock;

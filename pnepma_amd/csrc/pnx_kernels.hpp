@@ -118,6 +118,8 @@ hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2
 // pb, pa in {-1 (none), 0 (even), 1 (odd)}
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
+// dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
+hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);
 // I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps
 hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,
                                    const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,

@@ -11,14 +11,17 @@
 //     workgroup barriers between the half steps;
 //   * 2D/3D lanes (split by PN order): one operation per request per tick; for
 //     every (PN order, dimension) group and every one of the seven step slots
-//     one kernel runs over the work list of all requests in that
-//     slot (the grouped kernels).
+//     one kernel runs over the work list of all requests in that slot (the
+//     grouped kernels).  kGridTicks ticks are planned and uploaded at once, so
+//     the argument upload's latency is paid once per kGridTicks ticks.
 // Every request follows the operation sequence of pnx_forward_adjoint with
 // one beam (march_next), with the same device functions, so its results are
-// bitwise those of the request solved alone.  A lane keeps at most two ticks
-// in flight: tick t's argument blocks go through pinned staging buffer t mod 2,
-// reused once tick t-2 has completed; requests complete once the tick that
-// copied their results back has completed.
+// bitwise those of the request solved alone.  A lane keeps at most two
+// launch batches ("lane ticks") in flight: batch t's argument blocks go through
+// pinned staging buffer t mod 2, reused once batch t-2 has completed; requests
+// complete once the batch that copied their results back has completed.
+// Request inputs are staged at submit in a pinned block laid out like the
+// device buffers, so admission is one upload and one zero fill.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,33 +51,24 @@ namespace {
 constexpr size_t kAlign = 256;
 constexpr int kSlots = 7;      // fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
+constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 constexpr int kLanes = 3;      // 1D; 2D/3D of orders <= 5; 2D/3D of orders 7, 9
 
 int lane_of(int N, int dims) { return dims == 1 ? 0 : (N <= 5 ? 1 : 2); }
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
-// First-fit sub-allocator over one device arena.
-class Arena {
+// First-fit allocator of offsets within a range of `size` bytes.
+class FreeList {
  public:
-  int init(size_t bytes) {
-    if (hipMalloc(&base_, bytes) != hipSuccess) {
-      base_ = nullptr;
-      (void)hipGetLastError();
-      return -1;
-    }
-    size_ = bytes;
-    free_[0] = bytes;
-    return 0;
+  explicit FreeList(size_t size = 0) : size_(size) {
+    if (size) free_[0] = size;
   }
-  ~Arena() {
-    if (base_) (void)hipFree(base_);
-  }
-  // returns offset or SIZE_MAX
+  // returns the offset, or SIZE_MAX when no free range is large enough
   size_t alloc(size_t bytes) {
     bytes = align_up(bytes);
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second >= bytes) {
-        size_t off = it->first, len = it->second;
+        const size_t off = it->first, len = it->second;
         free_.erase(it);
         if (len > bytes) free_[off + bytes] = len - bytes;
         used_ += bytes;
@@ -100,14 +94,79 @@ class Arena {
       }
     }
   }
-  char* base() const { return base_; }
   size_t used() const { return used_; }
   size_t size() const { return size_; }
 
  private:
-  char* base_ = nullptr;
   size_t size_ = 0, used_ = 0;
   std::map<size_t, size_t> free_;
+};
+
+// The device arena: one allocation, sub-allocated first-fit.
+class Arena {
+ public:
+  int init(size_t bytes) {
+    if (hipMalloc(&base_, bytes) != hipSuccess) {
+      base_ = nullptr;
+      (void)hipGetLastError();
+      return -1;
+    }
+    fl_ = FreeList(bytes);
+    return 0;
+  }
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  size_t alloc(size_t bytes) { return fl_.alloc(bytes); }
+  void release(size_t off, size_t bytes) { fl_.release(off, bytes); }
+  char* base() const { return base_; }
+  size_t used() const { return fl_.used(); }
+
+ private:
+  char* base_ = nullptr;
+  FreeList fl_;
+};
+
+// Pinned host memory for request staging, in chunks that are sub-allocated
+// and kept: pinning and unpinning pages per request would cost milliseconds
+// in the submitting thread and in the scheduler.  Thread-safe.
+class PinnedPool {
+ public:
+  ~PinnedPool() {
+    for (auto& c : chunks_) (void)hipHostFree(c.base);
+  }
+  double* alloc(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    bytes = align_up(bytes);
+    for (auto& c : chunks_) {
+      const size_t off = c.fl.alloc(bytes);
+      if (off != SIZE_MAX) return reinterpret_cast<double*>(c.base + off);
+    }
+    const size_t cap = std::max(bytes, kChunk);
+    char* p = nullptr;
+    if (hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess) return nullptr;
+    chunks_.push_back(Chunk{p, cap, FreeList(cap)});
+    return reinterpret_cast<double*>(p + chunks_.back().fl.alloc(bytes));
+  }
+  void release(double* ptr, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    char* p = reinterpret_cast<char*>(ptr);
+    for (auto& c : chunks_)
+      if (p >= c.base && p < c.base + c.size) {
+        c.fl.release((size_t)(p - c.base), bytes);
+        return;
+      }
+  }
+
+ private:
+  static constexpr size_t kChunk = (size_t)256 << 20;
+  struct Chunk {
+    char* base;
+    size_t size;
+    FreeList fl;
+  };
+  std::mutex mu_;
+  std::vector<Chunk> chunks_;
 };
 
 // doubles each device buffer of a request needs
@@ -132,6 +191,46 @@ Sizes request_sizes(const Geometry& g, const pnx_desc& d) {
   return s;
 }
 
+// Offsets (doubles, each buffer 256-byte aligned) of a request's buffers in
+// its arena block.  The inputs come first, in the same layout as the pinned
+// host block they are staged in, so admission is one copy; the buffers that
+// start at zero follow, so admission clears them with one fill.
+struct Offsets {
+  long rec_f, rec_a, rec_d, rho, W, prof, mms, ibar, in_end;
+  long state, lam, rb, gen, seg, zero_end;
+  long ckpt, partial, I, total;
+};
+
+Offsets request_offsets(const Sizes& z) {
+  Offsets o{};
+  long p = 0;
+  auto take = [&](long n) {
+    const long q = p;
+    p += (long)(align_up((size_t)n * sizeof(double)) / sizeof(double));
+    return q;
+  };
+  o.rec_f = take(z.rec);
+  o.rec_a = take(z.rec);
+  o.rec_d = take(z.rec);
+  o.rho = take(z.rho);
+  o.W = take(z.W);
+  o.prof = take(z.prof);
+  o.mms = take(8 * 16);
+  o.ibar = take(8);
+  o.in_end = p;
+  o.state = take(z.state);
+  o.lam = take(z.state);
+  o.rb = take(z.rb);
+  o.gen = take(z.gen);
+  o.seg = take(z.seg);
+  o.zero_end = p;
+  o.ckpt = take(z.ckpt);
+  o.partial = take(z.partial);
+  o.I = take(8);
+  o.total = p;
+  return o;
+}
+
 int default_ckpt(int n) { return std::max(1, (int)std::ceil(std::sqrt((double)n))); }
 
 struct Request {
@@ -140,13 +239,16 @@ struct Request {
   Geometry g{};
   Sizes sz{};
   int lane = 0;
-  // host copies of the inputs (packed to the device layout at submit)
-  std::vector<double> h_rec, h_rho, h_W, h_prof, h_mms, h_ibar;
+  Offsets off_{};
+  // pinned host block: the inputs packed in the device layout (off_.in_end
+  // doubles), then the results (n_lines + rho_bar + gen doubles)
+  double* pinned = nullptr;
+  size_t pinned_bytes = 0;
   // outputs
   double* I_out = nullptr;
   double* rho_bar[8] = {};
   double* gen_out = nullptr;
-  double* pinned_out = nullptr;  // results land here, then go to the caller's arrays
+  double* pinned_out = nullptr;  // results land here (inside `pinned`), then go to the caller's arrays
   // device
   size_t off = 0, bytes = 0;
   double *state = nullptr, *lam = nullptr, *ckpt = nullptr, *seg = nullptr, *rho = nullptr, *W = nullptr,
@@ -196,6 +298,7 @@ struct pnx_server {
   std::atomic<bool> stop{false};
   std::thread worker;
   Lane lanes[kLanes];
+  PinnedPool pinned_pool;
   // coefficient buffers per order and direction
   std::map<int, double*> coef_f, coef_a;
   int64_t ticks = 0, completed = 0, peak_active = 0;
@@ -234,48 +337,33 @@ int fail_lane(pnx_server* s, const char* what) {
   return -1;
 }
 
-// Carve the request's buffers out of its arena block and enqueue uploads + zero
-// fills on its lane's stream.
+// Point the request's buffers into its arena block and enqueue the input
+// upload (one copy from the pinned block) and the zero fill on its lane's stream.
 int admit(pnx_server* s, Request& r) {
   hipStream_t st = s->lanes[r.lane].stream;
-  const Sizes& z = r.sz;
-  double* p = reinterpret_cast<double*>(s->arena.base() + r.off);
-  auto take = [&](long n) {
-    double* q = p;
-    p += (align_up(n * sizeof(double)) / sizeof(double));
-    return q;
-  };
-  r.state = take(z.state);
-  r.lam = take(z.state);
-  r.ckpt = take(z.ckpt);
-  r.seg = take(z.seg);
-  r.rho = take(z.rho);
-  r.W = take(z.W);
-  r.rec_f = take(z.rec);
-  r.rec_a = take(z.rec);
-  r.rec_d = take(z.rec);
-  r.prof = take(z.prof);
-  r.partial = take(z.partial);
-  r.rb = take(z.rb);
-  r.gen = take(z.gen);
-  r.mms = take(8 * 16);
-  r.I_dev = take(8);
-  r.ibar = take(8);
-  const long rec = z.rec;
-  auto up = [&](double* dst, const double* src, size_t n) {
-    return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess;
-  };
-  if (!up(r.rec_f, r.h_rec.data(), rec) || !up(r.rec_a, r.h_rec.data() + rec, rec) ||
-      !up(r.rec_d, r.h_rec.data() + 2 * rec, rec) || !up(r.rho, r.h_rho.data(), r.h_rho.size()) ||
-      !up(r.W, r.h_W.data(), r.h_W.size()) || !up(r.prof, r.h_prof.data(), r.h_prof.size()) ||
-      !up(r.mms, r.h_mms.data(), r.h_mms.size()) || !up(r.ibar, r.h_ibar.data(), r.h_ibar.size()))
+  const Offsets& o = r.off_;
+  double* base = reinterpret_cast<double*>(s->arena.base() + r.off);
+  r.rec_f = base + o.rec_f;
+  r.rec_a = base + o.rec_a;
+  r.rec_d = base + o.rec_d;
+  r.rho = base + o.rho;
+  r.W = base + o.W;
+  r.prof = base + o.prof;
+  r.mms = base + o.mms;
+  r.ibar = base + o.ibar;
+  r.state = base + o.state;
+  r.lam = base + o.lam;
+  r.rb = base + o.rb;
+  r.gen = base + o.gen;
+  r.seg = base + o.seg;
+  r.ckpt = base + o.ckpt;
+  r.partial = base + o.partial;
+  r.I_dev = base + o.I;
+  if (hipMemcpyAsync(base, r.pinned, (size_t)o.in_end * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
     return -1;
-  // the recompute writes whole segment slots; ghost slots are never read for
-  // values, but keep them finite
-  if (launch_fill(r.state, z.state, 0.0, st) != hipSuccess || launch_fill(r.lam, z.state, 0.0, st) != hipSuccess ||
-      launch_fill(r.rb, z.rb, 0.0, st) != hipSuccess || launch_fill(r.gen, z.gen, 0.0, st) != hipSuccess ||
-      launch_fill(r.seg, z.seg, 0.0, st) != hipSuccess)
-    return -1;
+  // state, lam, rho_bar, gen start at zero; the segment states are written
+  // before they are read, but their ghost slots are kept finite
+  if (launch_fill(r.state, o.zero_end - o.state, 0.0, st) != hipSuccess) return -1;
   r.pos = march_start();
   r.ops_left = march_ops(r.d.n_steps);
   return 0;
@@ -368,12 +456,19 @@ struct SlotWork {
   std::vector<int2> items;
 };
 
-struct TickPlan {
-  std::map<GroupKey, SlotWork[kSlots]> groups;      // grid lanes
-  std::map<int, std::vector<MarchArgs>> march;      // 1D lane, per order
+// One operation of every planned request of a grid lane.
+struct SubTick {
+  std::map<GroupKey, SlotWork[kSlots]> groups;
   std::vector<std::pair<Request*, int>> copies;     // (request, checkpoint index) copies before the slots
   std::vector<Request*> reduce;                     // forward finished: reduce intensities after slot 2
-  std::vector<Request*> copy_out;                   // results to copy back at the end of the tick
+};
+
+// What a lane enqueues at once: kGridTicks sub-ticks (grid lanes) or one
+// march slice per request (1D lane), with one argument upload.
+struct TickPlan {
+  std::vector<SubTick> sub;                         // grid lanes
+  std::map<int, std::vector<MarchArgs>> march;      // 1D lane, per order
+  std::vector<Request*> copy_out;                   // results to copy back at the end
 };
 
 void add_half(SlotWork& w, const HalfStepArgs& a) {
@@ -388,7 +483,7 @@ void add_dens(SlotWork& w, const DensArgs& a) {
 }
 
 // Plan the next operation of request r on a grid lane.
-void plan_grid(pnx_server* s, Request& r, TickPlan& plan) {
+void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
   const Geometry& g = r.g;
   SlotWork* w = plan.groups[GroupKey{r.d.N, g.L.dims}];
   const double h = g.dE / 2;
@@ -419,7 +514,7 @@ void plan_grid(pnx_server* s, Request& r, TickPlan& plan) {
     }
     case kOpFinal:
       add_dens(w[3], make_dens(s, r, 0, src, 0.5));
-      plan.copy_out.push_back(&r);
+      whole.copy_out.push_back(&r);
       break;
     default:
       break;
@@ -464,10 +559,11 @@ int ensure_dev_args(Lane& ln, size_t bytes) {
 }
 
 // Serialise the plan's argument blocks and work lists into staging buffer
-// `buf`, upload them, and enqueue the tick on the lane's stream.
+// `buf`, upload them, and enqueue the plan on the lane's stream.
 int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   hipStream_t st = ln.stream;
   struct Entry {
+    int sub;   // sub-tick (grid lanes)
     GroupKey key;
     int slot;  // -1: march block of order key.N
     size_t n_off, args_off, items_off;
@@ -476,24 +572,25 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   std::vector<Entry> entries;
   size_t off = 0;
   for (auto& kv : plan.march) {
-    Entry e{GroupKey{kv.first, 1}, -1, 0, 0, 0, (int)kv.second.size()};
+    Entry e{0, GroupKey{kv.first, 1}, -1, 0, 0, 0, (int)kv.second.size()};
     e.args_off = off;
     off = align_up(off + kv.second.size() * sizeof(MarchArgs));
     entries.push_back(e);
   }
-  for (auto& kv : plan.groups)
-    for (int sl = 0; sl < kSlots; ++sl) {
-      SlotWork& w = kv.second[sl];
-      if (w.items.empty()) continue;
-      Entry e{kv.first, sl, 0, 0, 0, (int)w.items.size()};
-      e.n_off = off;
-      off = align_up(off + sizeof(int));
-      e.args_off = off;
-      off = align_up(off + (sl == 3 ? w.dens.size() * sizeof(DensArgs) : w.half.size() * sizeof(HalfStepArgs)));
-      e.items_off = off;
-      off = align_up(off + w.items.size() * sizeof(int2));
-      entries.push_back(e);
-    }
+  for (int t = 0; t < (int)plan.sub.size(); ++t)
+    for (auto& kv : plan.sub[t].groups)
+      for (int sl = 0; sl < kSlots; ++sl) {
+        SlotWork& w = kv.second[sl];
+        if (w.items.empty()) continue;
+        Entry e{t, kv.first, sl, 0, 0, 0, (int)w.items.size()};
+        e.n_off = off;
+        off = align_up(off + sizeof(int));
+        e.args_off = off;
+        off = align_up(off + (sl == 3 ? w.dens.size() * sizeof(DensArgs) : w.half.size() * sizeof(HalfStepArgs)));
+        e.items_off = off;
+        off = align_up(off + w.items.size() * sizeof(int2));
+        entries.push_back(e);
+      }
   if (off == 0) off = kAlign;
   if (ensure_pinned(ln, buf, off) || ensure_dev_args(ln, off)) return -1;
   char* h = ln.pinned[buf];
@@ -503,7 +600,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       std::memcpy(h + e.args_off, v.data(), v.size() * sizeof(MarchArgs));
       continue;
     }
-    SlotWork& w = plan.groups[e.key][e.slot];
+    SlotWork& w = plan.sub[e.sub].groups[e.key][e.slot];
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
     if (e.slot == 3)
       std::memcpy(h + e.args_off, w.dens.data(), w.dens.size() * sizeof(DensArgs));
@@ -512,21 +609,14 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
     std::memcpy(h + e.items_off, w.items.data(), w.items.size() * sizeof(int2));
   }
   if (hipMemcpyAsync(ln.dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
-  for (auto& cp : plan.copies) {
-    Request& r = *cp.first;
-    const long stn = r.g.L.beam_stride;
-    if (hipMemcpyAsync(r.ckpt + (long)cp.second * stn, r.state, stn * sizeof(double), hipMemcpyDeviceToDevice, st) !=
-        hipSuccess)
-      return -1;
-  }
   for (const Entry& e : entries)
     if (e.slot < 0 &&
         launch_march(e.key.N, reinterpret_cast<const MarchArgs*>(ln.dev_args + e.args_off), e.n_items, st) !=
             hipSuccess)
       return -1;
-  auto launch_slot = [&](int slot) -> int {
+  auto launch_slot = [&](int t, int slot) -> int {
     for (const Entry& e : entries) {
-      if (e.slot != slot) continue;
+      if (e.slot != slot || e.sub != t) continue;
       const int grid = (e.n_items + 7) & ~7;  // one block per item, a multiple of 8 for the XCD mapping
       const int* nptr = reinterpret_cast<const int*>(ln.dev_args + e.n_off);
       const int2* items = reinterpret_cast<const int2*>(ln.dev_args + e.items_off);
@@ -545,17 +635,25 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
     }
     return 0;
   };
-  for (int sl = 0; sl < 3; ++sl)
-    if (launch_slot(sl)) return -1;
-  for (Request* r : plan.reduce) {
-    const Geometry& g = r->g;
-    if (launch_reduce_intensity(r->partial, g.partial_step, g.partial_beam, g.L.n_tiles, r->rec_f, g.rec_len,
-                                g.sigma_off, r->d.n_steps, r->d.n_lines, g.dE, g.L.dvol, 1, r->I_dev, st) !=
-        hipSuccess)
-      return -1;
+  for (int t = 0; t < (int)plan.sub.size(); ++t) {
+    SubTick& sp = plan.sub[t];
+    for (auto& cp : sp.copies) {
+      Request& r = *cp.first;
+      const long stn = r.g.L.beam_stride;
+      if (launch_copy(r.ckpt + (long)cp.second * stn, r.state, stn, st) != hipSuccess) return -1;
+    }
+    for (int sl = 0; sl < 3; ++sl)
+      if (launch_slot(t, sl)) return -1;
+    for (Request* r : sp.reduce) {
+      const Geometry& g = r->g;
+      if (launch_reduce_intensity(r->partial, g.partial_step, g.partial_beam, g.L.n_tiles, r->rec_f, g.rec_len,
+                                  g.sigma_off, r->d.n_steps, r->d.n_lines, g.dE, g.L.dvol, 1, r->I_dev, st) !=
+          hipSuccess)
+        return -1;
+    }
+    for (int sl = 3; sl < kSlots; ++sl)
+      if (launch_slot(t, sl)) return -1;
   }
-  for (int sl = 3; sl < kSlots; ++sl)
-    if (launch_slot(sl)) return -1;
   for (Request* r : plan.copy_out) {
     const long nI = r->d.n_lines, nrb = r->sz.rb, ngen = r->sz.gen;
     if (hipMemcpyAsync(r->pinned_out, r->I_dev, nI * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -582,16 +680,13 @@ void finish_request(pnx_server* s, Request& r) {
     }
   }
   if (r.gen_out) unpack_lines(r.g, out + nI + nrb, r.gen_out);
-  (void)hipHostFree(r.pinned_out);
+  s->pinned_pool.release(r.pinned, r.pinned_bytes);
+  r.pinned = nullptr;
   r.pinned_out = nullptr;
   s->arena.release(r.off, r.bytes);
   r.timing.done_s = s->now();
   r.timing.status = PNX_OK;
   r.done = true;
-  std::vector<double>().swap(r.h_rec);
-  std::vector<double>().swap(r.h_rho);
-  std::vector<double>().swap(r.h_W);
-  std::vector<double>().swap(r.h_prof);
 }
 
 // Records which of the lane's in-flight ticks have completed.
@@ -617,13 +712,25 @@ int step_lane(pnx_server* s, int li) {
   if (ln.ev_tick[buf] >= 0) return 0;  // tick - 2 still in flight
   TickPlan plan;
   bool any = false;
-  for (auto& r : ln.active) {
-    if (r->ops_left == 0) continue;
-    any = true;
-    if (li == 0)
-      plan_march(s, *r, plan);
-    else
-      plan_grid(s, *r, plan);
+  if (li == 0) {
+    for (auto& r : ln.active)
+      if (r->ops_left > 0) {
+        plan_march(s, *r, plan);
+        any = true;
+      }
+  } else {
+    for (int t = 0; t < kGridTicks; ++t) {
+      SubTick sub;
+      bool here = false;
+      for (auto& r : ln.active)
+        if (r->ops_left > 0) {
+          plan_grid(s, *r, sub, plan);
+          here = true;
+        }
+      if (!here) break;
+      plan.sub.push_back(std::move(sub));
+      any = true;
+    }
   }
   if (!any) return 0;
   if (launch_plan(ln, plan, buf)) return -1;
@@ -728,9 +835,7 @@ int64_t pnx_request_device_bytes(const pnx_desc* desc) {
   Geometry g;
   std::string err;
   if (make_geometry(d, g, err)) return set_error(PNX_ERR_ARG, err);
-  Sizes z = request_sizes(g, d);
-  // each buffer is aligned separately
-  return (int64_t)(z.total() * sizeof(double) + 20 * kAlign);
+  return (int64_t)(request_offsets(request_sizes(g, d)).total * sizeof(double));
 }
 
 static void release_server(pnx_server* s) {
@@ -744,8 +849,6 @@ static void release_server(pnx_server* s) {
     if (ln.dev_args) (void)hipFree(ln.dev_args);
     if (ln.stream && ln.own_stream) (void)hipStreamDestroy(ln.stream);
   }
-  for (auto& kv : s->all)
-    if (kv.second->pinned_out) (void)hipHostFree(kv.second->pinned_out);
   delete s;
 }
 
@@ -819,39 +922,40 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   if (rc) return set_error(rc, err);
   r->sz = request_sizes(r->g, r->d);
   r->lane = lane_of(r->d.N, r->g.L.dims);
-  r->bytes = (size_t)r->sz.total() * sizeof(double) + 20 * kAlign;
+  r->off_ = request_offsets(r->sz);
+  const Offsets& o = r->off_;
+  r->bytes = (size_t)o.total * sizeof(double);
   if ((int64_t)r->bytes > s->budget) return set_error(PNX_ERR_MEMORY, "request exceeds the server's HBM budget");
   const Geometry& g = r->g;
-  const long rec = r->sz.rec;
-  r->h_rec.resize(3 * rec);
-  std::memcpy(r->h_rec.data(), q->fwd, rec * sizeof(double));
-  std::memcpy(r->h_rec.data() + rec, q->adj, rec * sizeof(double));
-  std::memcpy(r->h_rec.data() + 2 * rec, q->dens, rec * sizeof(double));
-  r->h_rho.assign(g.rho_total, 0.0);
   for (int c = 0; c < 8; ++c) {
     if (!g.active(c)) continue;
     if (!q->rho[c]) return set_error(PNX_ERR_ARG, "densities missing for an updated class");
-    pack_class(g, c, q->rho[c], r->d.n_elements, r->h_rho.data() + g.rho_off[c]);
-  }
-  r->h_W.assign((size_t)r->d.n_lines * g.L.V, 0.0);
-  pack_lines(g, q->W, r->h_W.data());
-  r->h_prof.assign(g.prof_stride, 0.0);
-  for (int c = 0; c < 8; ++c)
-    if (odd_in(c, 0) && g.active(c) && (!q->ppos[c] || !q->mms[c]))
+    if (odd_in(c, 0) && (!q->ppos[c] || !q->mms[c]))
       return set_error(PNX_ERR_ARG, "beam data missing for an x-odd class");
-  pack_profiles(g, q->ppos, r->h_prof.data());
-  r->h_mms.assign(8 * 16, 0.0);
+  }
+  const size_t nout = (size_t)(r->d.n_lines + r->sz.rb + r->sz.gen);
+  const size_t nh = (size_t)o.in_end + nout;
+  r->pinned_bytes = nh * sizeof(double);
+  r->pinned = s->pinned_pool.alloc(r->pinned_bytes);
+  if (!r->pinned) return set_error(PNX_ERR_MEMORY, "pinned staging allocation failed");
+  double* h = r->pinned;
+  std::memset(h, 0, (size_t)o.in_end * sizeof(double));
+  const long rec = r->sz.rec;
+  std::memcpy(h + o.rec_f, q->fwd, rec * sizeof(double));
+  std::memcpy(h + o.rec_a, q->adj, rec * sizeof(double));
+  std::memcpy(h + o.rec_d, q->dens, rec * sizeof(double));
+  for (int c = 0; c < 8; ++c)
+    if (g.active(c)) pack_class(g, c, q->rho[c], r->d.n_elements, h + o.rho + g.rho_off[c]);
+  pack_lines(g, q->W, h + o.W);
+  pack_profiles(g, q->ppos, h + o.prof);
   for (int c = 0; c < 8; ++c)
     if (odd_in(c, 0) && g.active(c))
-      for (int m = 0; m < g.T.n[c]; ++m) r->h_mms[c * 16 + m] = q->mms[c][m];
-  r->h_ibar.assign(8, 0.0);
-  for (int l = 0; l < r->d.n_lines; ++l) r->h_ibar[l] = q->I_bar[l];
+      for (int m = 0; m < g.T.n[c]; ++m) h[o.mms + c * 16 + m] = q->mms[c][m];
+  for (int l = 0; l < r->d.n_lines; ++l) h[o.ibar + l] = q->I_bar[l];
+  r->pinned_out = h + o.in_end;
   r->I_out = q->I_out;
   for (int c = 0; c < 8; ++c) r->rho_bar[c] = q->rho_bar[c];
   r->gen_out = q->gen_out;
-  const size_t nout = (size_t)(r->d.n_lines + r->sz.rb + r->sz.gen);
-  if (hipHostMalloc(&r->pinned_out, nout * sizeof(double), hipHostMallocDefault) != hipSuccess)
-    return set_error(PNX_ERR_MEMORY, "pinned result buffer allocation failed");
   r->timing.arrival_s = q->arrival_s;
   {
     std::lock_guard<std::mutex> lk(s->mu);
