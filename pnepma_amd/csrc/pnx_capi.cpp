@@ -313,6 +313,37 @@ int forward_step(pnx_problem* p, int k, const double* src, double* dst, double* 
   return PNX_OK;
 }
 
+// Forward energy steps ka..kb with the odd half steps between consecutive
+// steps fused into one pass (the even classes they read do not change in
+// between): O1(ka), E(ka), [O3(ka) O1(ka+1)], E(ka+1), ..., E(kb), O3(kb).
+// Step k reads the state after step k-1 (src0 for k = ka) and writes dst(k);
+// dst(k) may equal dst(k-1) (in place).  partial(k) receives the intensity
+// partials of step k.
+template <class Dst, class Part>
+int forward_run(pnx_problem* p, int ka, int kb, const double* src0, Dst dst, Part partial) {
+  hipStream_t s = p->ctx->stream;
+  const int N = p->d.N, dims = p->L.dims;
+  const double h = p->g.dE / 2;
+  auto rec = [&](int k) { return p->rec_f.p + (long)k * p->g.rec_len; };
+  HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec(ka), h, src0, src0, dst(ka), nullptr), s));
+  for (int k = ka; k <= kb; ++k) {
+    const double* prev = k == ka ? src0 : dst(k - 1);
+    double* cur = dst(k);
+    HIP_TRY(launch_half_step(N, dims, 0, kForward, half_args(p, kForward, rec(k), p->g.dE, cur, prev, cur, partial(k)),
+                             s));
+    if (k < kb) {
+      HalfStepArgs a = half_args(p, kForward, rec(k), h, cur, cur, dst(k + 1) == cur ? nullptr : cur, nullptr);
+      a.rec2 = rec(k + 1);
+      a.dt2 = h;
+      a.to_dst2 = dst(k + 1);
+      HIP_TRY(launch_half_step2(N, dims, kForward, a, s));
+    } else {
+      HIP_TRY(launch_half_step(N, dims, 1, kForward, half_args(p, kForward, rec(k), h, cur, cur, cur, nullptr), s));
+    }
+  }
+  return PNX_OK;
+}
+
 int adjoint_step(pnx_problem* p, int i, double* lam) {
   hipStream_t s = p->ctx->stream;
   const double* rec = p->rec_a.p + (long)i * p->g.rec_len;
@@ -327,10 +358,11 @@ int adjoint_step(pnx_problem* p, int i, double* lam) {
 int enqueue_forward(pnx_problem* p) {
   hipStream_t s = p->ctx->stream;
   HIP_TRY(launch_fill(p->state.p, (long)p->L.beam_stride * p->d.n_beams, 0.0, s));
-  for (int k = 1; k <= p->d.n_steps; ++k) {
-    int rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step);
-    if (rc) return rc;
-  }
+  double* st = p->state.p;
+  int rc = forward_run(
+      p, 1, p->d.n_steps, st, [&](int) { return st; },
+      [&](int k) { return p->partial.p + (long)k * p->g.partial_step; });
+  if (rc) return rc;
   HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
                                   p->g.sigma_off, p->d.n_steps, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams,
                                   p->I_dev.p, s));
@@ -416,11 +448,13 @@ int enqueue_forward_adjoint(pnx_problem* p) {
   const int n = p->d.n_steps, K = p->g.K, N = p->d.N, dims = p->L.dims;
   int rc;
   HIP_TRY(launch_fill(p->state.p, st, 0.0, s));
-  for (int k = 1; k <= n; ++k) {
-    if ((k - 1) % K == 0)
-      HIP_TRY(hipMemcpyAsync(p->ckpt.p + (long)((k - 1) / K) * st, p->state.p, st * sizeof(double),
-                             hipMemcpyDeviceToDevice, s));
-    if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step))) return rc;
+  double* state = p->state.p;
+  auto part_k = [&](int k) { return p->partial.p + (long)k * p->g.partial_step; };
+  // one fused run per checkpoint segment: the state after each segment is stored
+  for (int sgi = 0; sgi < p->g.nck; ++sgi) {
+    const int k0 = sgi * K, k1 = std::min(n, k0 + K);
+    HIP_TRY(launch_copy(p->ckpt.p + (long)sgi * st, state, st, s));
+    if ((rc = forward_run(p, k0 + 1, k1, state, [&](int) { return state; }, part_k))) return rc;
   }
   HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
                                   p->g.sigma_off, n, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams, p->I_dev.p, s));
@@ -429,13 +463,12 @@ int enqueue_forward_adjoint(pnx_problem* p) {
   HIP_TRY(launch_fill(p->gen_b.p, (long)p->gen_b.n, 0.0, s));
   for (int sgi = p->g.nck - 1; sgi >= 0; --sgi) {
     const int k0 = sgi * K, k1 = std::min(n, k0 + K);
-    // recompute u_{k0+1} .. u_{k1} into the segment buffer
-    const double* src = p->ckpt.p + (long)sgi * st;
-    for (int k = k0 + 1; k <= k1; ++k) {
-      double* dst = p->seg.p + (long)(k - k0 - 1) * st;
-      if ((rc = forward_step(p, k, src, dst, p->partial.p))) return rc;  // partials of step 0 slot are scratch
-      src = dst;
-    }
+    // recompute u_{k0+1} .. u_{k1} into the segment buffer (the partials go to
+    // the scratch slot of step 0)
+    if ((rc = forward_run(
+             p, k0 + 1, k1, p->ckpt.p + (long)sgi * st, [&](int k) { return p->seg.p + (long)(k - k0 - 1) * st; },
+             [&](int) { return p->partial.p; })))
+      return rc;
     for (int i = k1; i > k0; --i) {
       const double* u = p->seg.p + (long)(i - k0 - 1) * st;
       HIP_TRY(launch_dens(N, dims, dens_args(p, i, u, 1.0), s));

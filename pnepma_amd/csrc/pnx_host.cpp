@@ -83,7 +83,12 @@ int make_geometry(const pnx_desc& d, Geometry& g, std::string& err) {
   }
   L.dvol = h[0] * h[1] * h[2];
   L.n_cells = L.Mx * L.My * L.Mz;
-  L.n_tiles = (L.n_cells + kBlock - 1) / kBlock;
+  {
+    const int tx = tile_x(dims), ty = kBlock / tx;
+    L.tiles_x = (L.Mx + tx - 1) / tx;
+    L.tiles_y = (L.My + ty - 1) / ty;
+    L.n_tiles = L.tiles_x * L.tiles_y * L.Mz;
+  }
   g.rec_len = d.n_elements + d.n_elements * (d.N + 1) + 1 + d.n_lines;
   L.rec_len = g.rec_len;
   g.sigma_off = d.n_elements + d.n_elements * (d.N + 1) + 1;

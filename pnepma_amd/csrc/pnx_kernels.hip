@@ -38,6 +38,22 @@ hipError_t half_step_grouped_n9(int dims, int parity, int mode, const HalfStepAr
 hipError_t dens_grouped_n9(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
                             hipStream_t s);
 
+hipError_t half_step2_n1(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t half_step2_grouped_n1(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
+hipError_t half_step2_n3(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t half_step2_grouped_n3(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
+hipError_t half_step2_n5(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t half_step2_grouped_n5(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
+hipError_t half_step2_n7(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t half_step2_grouped_n7(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
+hipError_t half_step2_n9(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
+hipError_t half_step2_grouped_n9(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
+
 hipError_t disc_n1(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n3(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n5(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
@@ -51,6 +67,29 @@ hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfSte
     case 5: return half_step_n5(dims, parity, mode, a, s);
     case 7: return half_step_n7(dims, parity, mode, a, s);
     case 9: return half_step_n9(dims, parity, mode, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return half_step2_n1(dims, mode, a, s);
+    case 3: return half_step2_n3(dims, mode, a, s);
+    case 5: return half_step2_n5(dims, mode, a, s);
+    case 7: return half_step2_n7(dims, mode, a, s);
+    case 9: return half_step2_n9(dims, mode, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
+                                     const int* n_items, int grid, hipStream_t s) {
+  switch (N) {
+    case 1: return half_step2_grouped_n1(dims, mode, args, items, n_items, grid, s);
+    case 3: return half_step2_grouped_n3(dims, mode, args, items, n_items, grid, s);
+    case 5: return half_step2_grouped_n5(dims, mode, args, items, n_items, grid, s);
+    case 7: return half_step2_grouped_n7(dims, mode, args, items, n_items, grid, s);
+    case 9: return half_step2_grouped_n9(dims, mode, args, items, n_items, grid, s);
   }
   return hipErrorInvalidValue;
 }

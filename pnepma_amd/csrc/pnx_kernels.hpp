@@ -23,11 +23,19 @@ struct Layout {
   double inv_h[3];
   double dvol;
   int n_cells;      // Mx*My*Mz unit cells
-  int n_tiles;      // ceil(n_cells / kBlock)
+  int tiles_x, tiles_y;  // tiles per row and per column of one z-plane
+  int n_tiles;      // tiles_x * tiles_y * Mz
   int rec_len;
 };
 
 constexpr int kBlock = 256;
+
+// A tile is the 256 cells one workgroup updates: a row segment of 256 cells in
+// 1D, a 32 x 8 patch of one z-plane in 2D and 3D (x fastest within the tile,
+// tiles x-fastest over the grid).  A square-ish patch fetches fewer rows of
+// the other parity twice than a run of two whole rows, and a wavefront still
+// covers whole 128-byte lines (two row segments of 32 doubles).
+constexpr int tile_x(int dims) { return dims == 1 ? kBlock : 32; }
 
 struct HalfStepArgs {
   Layout L;
@@ -47,6 +55,12 @@ struct HalfStepArgs {
   long partial_stride;    // doubles per beam in partial
   double dt;
   int n_beams;
+  // second odd half step of a fused pair (half_step2 kernels only): the last
+  // odd half step of one energy step followed by the first of the next
+  const double* rec2;     // energy record of the second update
+  double* to_dst2;        // state written by the second update (beam 0); to_dst then
+                          // receives the intermediate odd values, or is null
+  double dt2;
 };
 
 struct DensArgs {
@@ -108,6 +122,9 @@ struct DiscArgs {
 // Launchers (defined in pnx_kernels.hip); return hipSuccess or the launch error.
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s);
+// Two consecutive odd half steps in one pass (the neighbours of the even
+// classes do not change between them); see HalfStepArgs::rec2.
+hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, hipStream_t s);
 // Forms over a device work list of (argument index, tile) items with
 // *n_items entries: one block per item; `grid` is a multiple of 8 that is at
 // least *n_items.
@@ -115,6 +132,8 @@ hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const
                                     const int2* items, const int* n_items, int grid, hipStream_t s);
 hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
                                hipStream_t s);
+hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
+                                     const int* n_items, int grid, hipStream_t s);
 // pb, pa in {-1 (none), 0 (even), 1 (odd)}
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);

@@ -153,29 +153,28 @@ struct StepView {
   const double* rec;   // energy record
   double* partial;     // intensity partials of this step (forward even half step)
   double dt;
+  const double* rec2;  // fused odd pair: record, output and step of the second update
+  double* dst2;
+  double dt2;
 };
 
 __device__ __forceinline__ StepView view_of(const HalfStepArgs& a) {
-  return StepView{a.from, a.to_src, a.to_dst, a.rec, a.partial, a.dt};
+  return StepView{a.from, a.to_src, a.to_dst, a.rec, a.partial, a.dt, a.rec2, a.to_dst2, a.dt2};
 }
 
-template <int N, int DIM, int C, int MODE, int PAR>
-__device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
-                                             gcd __restrict__ from, gcd src, gd dst, int i, int j, int k, long p,
-                                             int beam, double* contrib) {
-  // src and dst are the same buffer for in-place steps: the own values are
-  // read before they are overwritten, so neither pointer is restrict.
+// One update of class C at point p (right_hand_side! + update!,
+// solver_core.jl:92-106, solver.jl:122-124): own values u -> un, in registers.
+template <int N, int DIM, int C, int MODE>
+__device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+                                            gcd __restrict__ from, const double* u, double* un, int i, int j, int k,
+                                            long p, int beam) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
   const long V = L.V;
   const int ne = L.n_e;
-  gcd uc = src + L.cls_off[C];
-  double u[NC], du[NC];
+  double du[NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) {
-    u[m] = uc[m * V + p];
-    du[m] = 0.0;
-  }
+  for (int m = 0; m < NC; ++m) du[m] = 0.0;
   const double gfac = rec[ne + ne * (N + 1)];
   stencil<N, DIM, C, MODE == kForward>(L, from, glb(a.coef), u, du, i, j, k, p, gfac,
                                        glb(a.prof) + beam * a.prof_stride, glb(a.mms));
@@ -197,7 +196,6 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restri
       for (int r = 0; r < L.n_lines; ++r) q0 -= sig[r] * Wp[r * V + p] * ib[r];
     }
   }
-  gd out = dst + L.cls_off[C];
   static_for<0, NC>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     constexpr int l = Tab<N>::t.l[C][I];
@@ -210,12 +208,55 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restri
     }
     t = t * invS;
     t = t * phi[l];
-    const double un = u[I] + dt * t;
-    out[I * V + p] = un;
-    if constexpr (MODE == kForward && PAR == 0 && C == 0 && I == 0) {
-      for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = glb(a.W)[r * V + p] * un;
-    }
+    un[I] = u[I] + dt * t;
   });
+}
+
+template <int N, int DIM, int C, int MODE, int PAR>
+__device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+                                             gcd __restrict__ from, gcd src, gd dst, int i, int j, int k, long p,
+                                             int beam, double* contrib) {
+  // src and dst are the same buffer for in-place steps: the own values are
+  // read before they are overwritten, so neither pointer is restrict.
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  gcd uc = src + L.cls_off[C];
+  double u[NC], un[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) u[m] = uc[m * V + p];
+  update_core<N, DIM, C, MODE>(a, rec, dt, from, u, un, i, j, k, p, beam);
+  gd out = dst + L.cls_off[C];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) out[m * V + p] = un[m];
+  if constexpr (MODE == kForward && PAR == 0 && C == 0) {
+    for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = glb(a.W)[r * V + p] * un[0];
+  }
+}
+
+// Two consecutive odd-parity updates of class C at point p: the even classes
+// they read do not change in between, and the intermediate values stay in
+// registers (stored to dst only when dst is not null).
+template <int N, int DIM, int C, int MODE>
+__device__ __forceinline__ void update_class2(const HalfStepArgs& a, const StepView& v, gcd __restrict__ from, gcd src,
+                                              gd dst, gd dst2, int i, int j, int k, long p, int beam) {
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  gcd uc = src + L.cls_off[C];
+  double u[NC], u1[NC], u2[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) u[m] = uc[m * V + p];
+  update_core<N, DIM, C, MODE>(a, glb(v.rec), v.dt, from, u, u1, i, j, k, p, beam);
+  if (dst) {
+    gd out = dst + L.cls_off[C];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) out[m * V + p] = u1[m];
+  }
+  update_core<N, DIM, C, MODE>(a, glb(v.rec2), v.dt2, from, u1, u2, i, j, k, p, beam);
+  gd out2 = dst2 + L.cls_off[C];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) out2[m * V + p] = u2[m];
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -281,18 +322,26 @@ __device__ __forceinline__ int grouped_first() {
   return (G & 7) == 0 ? (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3) : blockIdx.x;
 }
 
+// Cell (i, j, k) of this thread in tile `tile` (tile_x(DIM) x 256/tile_x(DIM)
+// cells of one z-plane); false for threads past the grid's edge.
+template <int DIM>
+__device__ __forceinline__ bool tile_cell(const Layout& L, int tile, int& i, int& j, int& k) {
+  constexpr int TX = tile_x(DIM), TY = kBlock / TX;
+  const int tx = tile % L.tiles_x;
+  const int r = tile / L.tiles_x;
+  const int ty = r % L.tiles_y;
+  k = r / L.tiles_y;
+  i = tx * TX + (int)threadIdx.x % TX;
+  j = ty * TY + (int)threadIdx.x / TX;
+  return i < L.Mx && j < L.My;
+}
+
 template <int N, int DIM, int PAR, int MODE>
 __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const StepView& v, int beam, int tile) {
   const Layout& L = a.L;
-  const int q = tile * kBlock + threadIdx.x;
-  const bool valid = q < L.n_cells;
-  int i = 0, j = 0, k = 0;
-  if (valid) {
-    i = q % L.Mx;
-    const int r = q / L.Mx;
-    j = r % L.My;
-    k = r / L.My;
-  }
+  int i, j, k;
+  const bool valid = tile_cell<DIM>(L, tile, i, j, k);
+  if (!valid) i = j = k = 0;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
   double contrib[kMaxLines];
@@ -346,6 +395,47 @@ __global__ __launch_bounds__(kBlock) void half_step_grouped(const HalfStepArgs* 
   const int2 w = items[it];
   const HalfStepArgs& a = args[w.x];
   half_step_body<N, DIM, PAR, MODE>(a, view_of(a), 0, w.y);
+}
+
+// Fused pair of odd half steps: the last odd half step of energy step k
+// (record v.rec, step v.dt, intermediate output v.dst or none) and the first
+// of step k+1 (v.rec2, v.dt2, output v.dst2).
+template <int N, int DIM, int MODE>
+__device__ __forceinline__ void half_step2_body(const HalfStepArgs& a, const StepView& v, int beam, int tile) {
+  const Layout& L = a.L;
+  int i, j, k;
+  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const long bo = (long)beam * L.beam_stride;
+  gd dst = v.dst ? glb(v.dst) + bo : nullptr;
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (parity_of(C) == 1 && class_active(C, DIM) && ncls<N, C>() > 0) {
+      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                              (DIM >= 3 && odd_in(C, 2) && k == 0));
+      if (interior)
+        update_class2<N, DIM, C, MODE>(a, v, glb(v.from) + bo, glb(v.src) + bo, dst, glb(v.dst2) + bo, i, j, k, p,
+                                       beam);
+    }
+  });
+}
+
+template <int N, int DIM, int MODE>
+__global__ __launch_bounds__(kBlock) void half_step2_kernel(HalfStepArgs a) {
+  int beam, tile;
+  block_coords(a.L.n_tiles, a.n_beams, beam, tile);
+  half_step2_body<N, DIM, MODE>(a, view_of(a), beam, tile);
+}
+
+template <int N, int DIM, int MODE>
+__global__ __launch_bounds__(kBlock) void half_step2_grouped(const HalfStepArgs* __restrict__ args,
+                                                             const int2* __restrict__ items,
+                                                             const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const HalfStepArgs& a = args[w.x];
+  half_step2_body<N, DIM, MODE>(a, view_of(a), 0, w.y);
 }
 
 // integrate_adjoint_densities! (pn_solver/pn_intensities.jl:129-187) for one
@@ -409,12 +499,8 @@ template <int N, int DIM>
 __device__ __forceinline__ void dens_body(const DensArgs& a, const double* u, const double* rec, double int_fac,
                                           int beam, int tile) {
   const Layout& L = a.L;
-  const int q = tile * kBlock + threadIdx.x;
-  if (q >= L.n_cells) return;
-  const int i = q % L.Mx;
-  const int r = q / L.Mx;
-  const int j = r % L.My;
-  const int k = r / L.My;
+  int i, j, k;
+  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
   static_for<0, 8>([&](auto Cc) {
@@ -610,12 +696,8 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
 template <int N, int DIM, int PB, int PA>
 __device__ __forceinline__ void disc_body(const DiscArgs& a, int beam, int tile) {
   const Layout& L = a.L;
-  const int q = tile * kBlock + threadIdx.x;
-  if (q >= L.n_cells) return;
-  const int i = q % L.Mx;
-  const int r = q / L.Mx;
-  const int j = r % L.My;
-  const int k = r / L.My;
+  int i, j, k;
+  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   static_for<0, 8>([&](auto Cc) {
     constexpr int C = decltype(Cc)::value;
@@ -702,6 +784,34 @@ hipError_t half_step_grouped_order(int dims, int parity, int mode, const HalfSte
     PNX_CASE(1, 0, 0) PNX_CASE(1, 0, 1) PNX_CASE(1, 1, 0) PNX_CASE(1, 1, 1)
     PNX_CASE(2, 0, 0) PNX_CASE(2, 0, 1) PNX_CASE(2, 1, 0) PNX_CASE(2, 1, 1)
     PNX_CASE(3, 0, 0) PNX_CASE(3, 0, 1) PNX_CASE(3, 1, 0) PNX_CASE(3, 1, 1)
+#undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t half_step2_order(int dims, int mode, const HalfStepArgs& a, hipStream_t s) {
+  const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
+  switch (dims * 2 + mode) {
+#define PNX_CASE(D, M) \
+  case D * 2 + M: hipLaunchKernelGGL((half_step2_kernel<N, D, M>), grid, block, 0, s, a); break;
+    PNX_CASE(1, 0) PNX_CASE(1, 1) PNX_CASE(2, 0) PNX_CASE(2, 1) PNX_CASE(3, 0) PNX_CASE(3, 1)
+#undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t half_step2_grouped_order(int dims, int mode, const HalfStepArgs* args, const int2* items,
+                                    const int* n_items, int grid, hipStream_t s) {
+  switch (dims * 2 + mode) {
+#define PNX_CASE(D, M)                                                                                    \
+  case D * 2 + M:                                                                                         \
+    hipLaunchKernelGGL((half_step2_grouped<N, D, M>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); \
+    break;
+    PNX_CASE(1, 0) PNX_CASE(1, 1) PNX_CASE(2, 0) PNX_CASE(2, 1) PNX_CASE(3, 0) PNX_CASE(3, 1)
 #undef PNX_CASE
     default: return hipErrorInvalidValue;
   }

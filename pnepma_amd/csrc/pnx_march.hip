@@ -56,9 +56,17 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
       const double* rec = m.rec_f + (long)o.step * m.rec_len;
       // the recompute's intensity partials go to the scratch slot of step 0
       double* part = o.kind == kOpFwd ? m.partial + (long)o.step * m.partial_step : m.partial;
-      half_pass<N, 1, kForward>(m.fwd, StepView{src, src, dst, rec, nullptr, h});
+      if (o.first) half_pass<N, 1, kForward>(m.fwd, StepView{src, src, dst, rec, nullptr, h});
       half_pass<N, 0, kForward>(m.fwd, StepView{dst, src, dst, rec, part, dE});
-      half_pass<N, 1, kForward>(m.fwd, StepView{dst, dst, dst, rec, nullptr, h});
+      if (o.fuse) {
+        // last odd half step of this step and the first of the next, in one pass
+        double* dst2 = march_buffer(m, o.dst2_buf, o.dst2_idx, st);
+        const StepView v{dst, dst, dst2 == dst ? nullptr : dst, rec, nullptr, h, rec + m.rec_len, dst2, h};
+        for (int t = 0; t < L.n_tiles; ++t) half_step2_body<N, 1, kForward>(m.fwd, v, 0, t);
+        __syncthreads();
+      } else {
+        half_pass<N, 1, kForward>(m.fwd, StepView{dst, dst, dst, rec, nullptr, h});
+      }
       if (o.reduce)
         reduce_intensity_body(m.partial, m.partial_step, L.n_tiles, m.rec_f, m.rec_len, m.sigma_off, m.n, L.n_lines,
                               dE, L.dvol, m.I_out);

@@ -10,6 +10,12 @@
 //   i = k1..k0+1 (density pullback on the recomputed u_i, then one adjoint
 //   step); finally the density pullback at u_0 with weight 0.5.
 // Total: 3n + 1 operations.
+//
+// Within one run of consecutive forward steps (one checkpoint segment of the
+// forward pass, or one recompute) the last odd half step of step k and the
+// first of step k+1 run as one fused pass: an operation of such a run starts
+// with the first odd half step only if it is the run's first (`first`), and
+// ends with the fused pair if it is not the run's last (`fuse`).
 #pragma once
 
 #include "pnx_kernels.hpp"
@@ -30,6 +36,9 @@ struct MarchOp {
   int ckpt_copy;  // checkpoint slot that receives the state before a forward step, or -1
   int src_buf, src_idx;
   int dst_buf, dst_idx;
+  int dst2_buf, dst2_idx;  // fused pair: output of the next step's first odd half step
+  int first;      // run the step's first odd half step (first step of a run)
+  int fuse;       // end with the fused pair instead of the last odd half step
   int reduce;     // forward finished after this step: form the intensities
 };
 
@@ -39,13 +48,15 @@ constexpr int march_ops(int n) { return 3 * n + 1; }
 
 // Returns the operation at position s and advances s (n steps, checkpoints every K, nck of them).
 constexpr MarchOp march_next(MarchPos& s, int n, int K, int nck) {
-  MarchOp o{kOpNone, 0, -1, kBufState, 0, kBufState, 0, 0};
+  MarchOp o{kOpNone, 0, -1, kBufState, 0, kBufState, 0, kBufState, 0, 0, 0, 0};
   switch (s.stage) {
     case kStFwd: {
       const int k = s.k;
       o.kind = kOpFwd;
       o.step = k;
       o.ckpt_copy = (k - 1) % K == 0 ? (k - 1) / K : -1;
+      o.first = (k - 1) % K == 0;
+      o.fuse = k < n && k % K != 0;
       if (k == n) {
         o.reduce = 1;
         s.stage = kStRecompute;
@@ -69,6 +80,10 @@ constexpr MarchOp march_next(MarchPos& s, int n, int K, int nck) {
       }
       o.dst_buf = kBufSeg;
       o.dst_idx = k - k0 - 1;
+      o.dst2_buf = kBufSeg;
+      o.dst2_idx = k - k0;
+      o.first = k == k0 + 1;
+      o.fuse = k < k1;
       if (k == k1) {
         s.stage = kStReverse;
         s.i = k1;

@@ -49,7 +49,9 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr int kSlots = 7;      // fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd
+// step slots: fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd, fused fwd odd pair
+constexpr int kSlots = 8;
+constexpr int kSlotPair = 7;
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 constexpr int kLanes = 3;      // 1D; 2D/3D of orders <= 5; 2D/3D of orders 7, 9
@@ -498,9 +500,18 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       const double* rec = r.rec_f + (long)o.step * g.rec_len;
       // the recompute's intensity partials go to the scratch slot of step 0
       double* part = o.kind == kOpFwd ? r.partial + (long)o.step * g.partial_step : r.partial;
-      add_half(w[0], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
+      if (o.first) add_half(w[0], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
       add_half(w[1], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part));
-      add_half(w[2], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
+      if (o.fuse) {
+        double* dst2 = buffer_of(r, o.dst2_buf, o.dst2_idx);
+        HalfStepArgs a = make_half(s, r, kForward, rec, h, dst, dst, dst2 == dst ? nullptr : dst, nullptr);
+        a.rec2 = rec + g.rec_len;
+        a.dt2 = h;
+        a.to_dst2 = dst2;
+        add_half(w[kSlotPair], a);
+      } else {
+        add_half(w[2], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
+      }
       if (o.reduce) plan.reduce.push_back(&r);
       break;
     }
@@ -624,6 +635,10 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       if (slot == 3) {
         err = launch_dens_grouped(e.key.N, e.key.dims, reinterpret_cast<const DensArgs*>(ln.dev_args + e.args_off),
                                   items, nptr, grid, st);
+      } else if (slot == kSlotPair) {
+        err = launch_half_step2_grouped(e.key.N, e.key.dims, kForward,
+                                        reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
+                                        grid, st);
       } else {
         const int mode = slot < 3 ? kForward : kAdjoint;
         const int parity = (slot == 1 || slot == 5) ? 0 : 1;
@@ -642,7 +657,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       const long stn = r.g.L.beam_stride;
       if (launch_copy(r.ckpt + (long)cp.second * stn, r.state, stn, st) != hipSuccess) return -1;
     }
-    for (int sl = 0; sl < 3; ++sl)
+    for (int sl : {0, 1, 2, kSlotPair})
       if (launch_slot(t, sl)) return -1;
     for (Request* r : sp.reduce) {
       const Geometry& g = r->g;
@@ -651,7 +666,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
           hipSuccess)
         return -1;
     }
-    for (int sl = 3; sl < kSlots; ++sl)
+    for (int sl : {3, 4, 5, 6})
       if (launch_slot(t, sl)) return -1;
   }
   for (Request* r : plan.copy_out) {

@@ -81,10 +81,13 @@ def test_raster2d_batch_properties():
     dp50 = DeviceProblem(p, si, ckpt_every=50)
     I50, rb50, g50 = dp50.forward_adjoint(ib)
     dp50.close()
-    assert np.array_equal(I16, I50)
-    for c in range(8):
-        assert np.array_equal(rb16[c], rb50[c])
-    assert np.array_equal(g16, g50)
+    # the checkpoint interval only moves the boundaries of the fused odd-pair
+    # runs; print how far apart the two results are, then bound it
+    e_I = rel_err(I16, I50)
+    e_rb = max(rel_err(rb16[c], rb50[c]) for c in range(8) if np.any(rb50[c]))
+    e_g = rel_err(g16, g50)
+    print(f"ckpt 16 vs 50: I {e_I:.3e} rho_bar {e_rb:.3e} gen {e_g:.3e}", flush=True)
+    assert e_I < 1e-13 and e_rb < 1e-12 and e_g < 1e-13
     # a beam alone equals the same beam in the batch
     q = _single_beam(p, 7)
     Iq = DeviceProblem(q, solve_inputs(q, mat, det), adjoint=False).forward()[0]
