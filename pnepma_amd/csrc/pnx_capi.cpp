@@ -469,10 +469,19 @@ int enqueue_forward_adjoint(pnx_problem* p) {
              p, k0 + 1, k1, p->ckpt.p + (long)sgi * st, [&](int k) { return p->seg.p + (long)(k - k0 - 1) * st; },
              [&](int) { return p->partial.p; })))
       return rc;
+    // per reverse step i: dens(i) + A1(i) fused, then AE(i), A3(i)
+    const double h = p->g.dE / 2;
+    auto rec_a = [&](int i) { return p->rec_a.p + (long)i * p->g.rec_len; };
     for (int i = k1; i > k0; --i) {
       const double* u = p->seg.p + (long)(i - k0 - 1) * st;
-      HIP_TRY(launch_dens(N, dims, dens_args(p, i, u, 1.0), s));
-      if ((rc = adjoint_step(p, i, p->lam.p))) return rc;
+      RevArgs ra{};
+      ra.adj = half_args(p, kAdjoint, rec_a(i), h, p->lam.p, p->lam.p, p->lam.p, nullptr);
+      ra.dens = dens_args(p, i, u, 1.0);
+      HIP_TRY(launch_rev(N, dims, ra, s));
+      HIP_TRY(launch_half_step(N, dims, 0, kAdjoint,
+                               half_args(p, kAdjoint, rec_a(i), p->g.dE, p->lam.p, p->lam.p, p->lam.p, nullptr), s));
+      HIP_TRY(launch_half_step(N, dims, 1, kAdjoint,
+                               half_args(p, kAdjoint, rec_a(i), h, p->lam.p, p->lam.p, p->lam.p, nullptr), s));
     }
   }
   // final accumulation at u_0 = 0 with weight 0.5 (pn_intensities.jl:402-404)

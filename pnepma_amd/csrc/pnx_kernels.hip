@@ -54,6 +54,22 @@ hipError_t half_step2_n9(int dims, int mode, const HalfStepArgs& a, hipStream_t 
 hipError_t half_step2_grouped_n9(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
                                   int grid, hipStream_t s);
 
+hipError_t rev_n1(int dims, const RevArgs& a, hipStream_t s);
+hipError_t rev_grouped_n1(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s);
+hipError_t rev_n3(int dims, const RevArgs& a, hipStream_t s);
+hipError_t rev_grouped_n3(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s);
+hipError_t rev_n5(int dims, const RevArgs& a, hipStream_t s);
+hipError_t rev_grouped_n5(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s);
+hipError_t rev_n7(int dims, const RevArgs& a, hipStream_t s);
+hipError_t rev_grouped_n7(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s);
+hipError_t rev_n9(int dims, const RevArgs& a, hipStream_t s);
+hipError_t rev_grouped_n9(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s);
+
 hipError_t disc_n1(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n3(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n5(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
@@ -90,6 +106,29 @@ hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepAr
     case 5: return half_step2_grouped_n5(dims, mode, args, items, n_items, grid, s);
     case 7: return half_step2_grouped_n7(dims, mode, args, items, n_items, grid, s);
     case 9: return half_step2_grouped_n9(dims, mode, args, items, n_items, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s) {
+  switch (N) {
+    case 1: return rev_n1(dims, a, s);
+    case 3: return rev_n3(dims, a, s);
+    case 5: return rev_n5(dims, a, s);
+    case 7: return rev_n7(dims, a, s);
+    case 9: return rev_n9(dims, a, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                              hipStream_t s) {
+  switch (N) {
+    case 1: return rev_grouped_n1(dims, args, items, n_items, grid, s);
+    case 3: return rev_grouped_n3(dims, args, items, n_items, grid, s);
+    case 5: return rev_grouped_n5(dims, args, items, n_items, grid, s);
+    case 7: return rev_grouped_n7(dims, args, items, n_items, grid, s);
+    case 9: return rev_grouped_n9(dims, args, items, n_items, grid, s);
   }
   return hipErrorInvalidValue;
 }

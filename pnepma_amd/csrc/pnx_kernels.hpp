@@ -86,6 +86,12 @@ struct DensArgs {
   int n_beams;
 };
 
+// Fused reverse pass of one reference-adjoint step (see rev_body).
+struct RevArgs {
+  HalfStepArgs adj;
+  DensArgs dens;
+};
+
 // Exact adjoint of one forward energy step, one fused pass per half step.
 // In the cell of every thread the pass first applies the transposed stencil of
 // the half step that was back-propagated last (parity PB points, reading its
@@ -134,6 +140,11 @@ hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2
                                hipStream_t s);
 hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
                                      const int* n_items, int grid, hipStream_t s);
+// Reverse pass of step i: density pullback of step i and the first odd
+// adjoint half step of step i in one sweep.
+hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s);
+hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                              hipStream_t s);
 // pb, pa in {-1 (none), 0 (even), 1 (odd)}
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);

@@ -442,10 +442,11 @@ __global__ __launch_bounds__(kBlock) void half_step2_grouped(const HalfStepArgs*
 // reverse step: du of every class without the exponential factor, then
 // rho_bar += lambda (s du + c u) dvol dE int_fac, plus the x-ray generation
 // pullback of integrate_step_and_add_pullback! at eee points.
+// lam holds the adjoint values of class C at this point (registers).
 template <int N, int DIM, int C>
-__device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ rec, double int_fac,
-                                           gcd __restrict__ u_b, gcd __restrict__ lam_b, int i, int j, int k, long p,
-                                           int beam) {
+__device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ rec, double int_fac,
+                                          gcd __restrict__ u_b, const double* lam, int i, int j, int k, long p,
+                                          int beam) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
   const long V = L.V;
@@ -462,12 +463,10 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ r
                           glb(a.mms));
   double S, Cl[N + 1];
   mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
-  gcd __restrict__ lc = lam_b + L.cls_off[C];
-  double lam[NC], t[NC];
+  double t[NC];
   static_for<0, NC>([&](auto Ic) {
     constexpr int I = decltype(Ic)::value;
     constexpr int l = Tab<N>::t.l[C][I];
-    lam[I] = lc[I * V + p];
     t[I] = ((-du[I]) - Cl[l] * u[I]) / S;
   });
   gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
@@ -493,6 +492,18 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ r
       g[r * V + p] += u[0] * integral;
     }
   }
+}
+
+template <int N, int DIM, int C>
+__device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ rec, double int_fac,
+                                           gcd __restrict__ u_b, gcd __restrict__ lam_b, int i, int j, int k, long p,
+                                           int beam) {
+  constexpr int NC = ncls<N, C>();
+  gcd __restrict__ lc = lam_b + a.L.cls_off[C];
+  double lam[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) lam[m] = lc[m * a.L.V + p];
+  dens_core<N, DIM, C>(a, rec, int_fac, u_b, lam, i, j, k, p, beam);
 }
 
 template <int N, int DIM>
@@ -528,6 +539,73 @@ __global__ __launch_bounds__(kBlock) void dens_grouped(const DensArgs* __restric
   const int2 w = items[it];
   const DensArgs& a = args[w.x];
   dens_body<N, DIM>(a, a.u, a.rec, a.int_fac, 0, w.y);
+}
+
+// Fused reverse pass of step i (reference adjoint): the density pullback of
+// step i and the first odd adjoint half step of step i in one sweep.  The
+// pullback reads lambda only at its own point and the odd half step reads
+// only the even classes of lambda at neighbouring points, so each thread
+// carries its odd-class lambda from the one to the other in registers.  (Also
+// folding in the previous step's last odd half step saves more bytes, but the
+// kernel then needs every register of the SIMD and runs at occupancy 1,
+// which measured slower.)
+//   a.adj:  from = to_src = to_dst = lambda; rec = adjoint record of step i, dt = dE/2
+//   a.dens: u = u_i, rec = density record of step i, int_fac
+template <int N, int DIM, int C>
+__device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam_b, gcd __restrict__ u_b, int i, int j,
+                                          int k, long p, int beam) {
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.adj.L;
+  const long V = L.V;
+  gcd lc = lam_b + L.cls_off[C];
+  double l0[NC], l1[NC];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) l0[m] = lc[m * V + p];
+  dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
+  update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+  gd out = glb(a.adj.to_dst) + (long)beam * L.beam_stride + L.cls_off[C];
+#pragma unroll
+  for (int m = 0; m < NC; ++m) out[m * V + p] = l1[m];
+}
+
+template <int N, int DIM>
+__device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
+  const Layout& L = a.adj.L;
+  int i, j, k;
+  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const long bo = (long)beam * L.beam_stride;
+  gcd lam_b = glb(a.adj.from) + bo;
+  gcd u_b = glb(a.dens.u) + bo;
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (class_active(C, DIM) && ncls<N, C>() > 0) {
+      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
+                              (DIM >= 3 && odd_in(C, 2) && k == 0));
+      if (interior) {
+        if constexpr (parity_of(C) == 1)
+          rev_class<N, DIM, C>(a, lam_b, u_b, i, j, k, p, beam);
+        else
+          dens_class<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
+      }
+    }
+  });
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs a) {
+  int beam, tile;
+  block_coords(a.adj.L.n_tiles, a.adj.n_beams, beam, tile);
+  rev_body<N, DIM>(a, beam, tile);
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void rev_grouped(const RevArgs* __restrict__ args, const int2* __restrict__ items,
+                                                      const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  rev_body<N, DIM>(args[w.x], 0, w.y);
 }
 
 // ----------------------------------------------------------- exact adjoint
@@ -813,6 +891,30 @@ hipError_t half_step2_grouped_order(int dims, int mode, const HalfStepArgs* args
     break;
     PNX_CASE(1, 0) PNX_CASE(1, 1) PNX_CASE(2, 0) PNX_CASE(2, 1) PNX_CASE(3, 0) PNX_CASE(3, 1)
 #undef PNX_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
+  const dim3 grid(a.adj.L.n_tiles * a.adj.n_beams), block(kBlock);
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((rev_kernel<N, 2>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((rev_kernel<N, 3>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t rev_grouped_order(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                             hipStream_t s) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((rev_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 2: hipLaunchKernelGGL((rev_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 3: hipLaunchKernelGGL((rev_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -9,6 +9,13 @@ hipError_t half_step_n5(int dims, int parity, int mode, const HalfStepArgs& a, h
 
 hipError_t dens_n5(int dims, const DensArgs& a, hipStream_t s) { return dens_order<5>(dims, a, s); }
 
+hipError_t rev_n5(int dims, const RevArgs& a, hipStream_t s) { return rev_order<5>(dims, a, s); }
+
+hipError_t rev_grouped_n5(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
+                           hipStream_t s) {
+  return rev_grouped_order<5>(dims, args, items, n_items, grid, s);
+}
+
 hipError_t half_step2_n5(int dims, int mode, const HalfStepArgs& a, hipStream_t s) {
   return half_step2_order<5>(dims, mode, a, s);
 }

@@ -32,6 +32,21 @@ __device__ __forceinline__ void dens_pass(const DensArgs& a, const double* u, co
   __syncthreads();
 }
 
+// Fused density pullback + first odd adjoint half step of one reverse step.
+template <int N>
+__device__ __forceinline__ void rev_pass(const MarchArgs& m, const double* u, const double* rec_a, const double* rec_d) {
+  RevArgs ra;
+  ra.adj = m.adj;
+  ra.adj.rec = rec_a;
+  ra.adj.dt = m.dens.dE / 2;
+  ra.dens = m.dens;
+  ra.dens.u = u;
+  ra.dens.rec = rec_d;
+  ra.dens.int_fac = 1.0;
+  for (int t = 0; t < ra.adj.L.n_tiles; ++t) rev_body<N, 1>(ra, 0, t);
+  __syncthreads();
+}
+
 __device__ __forceinline__ double* march_buffer(const MarchArgs& m, int buf, int idx, long st) {
   if (buf == kBufCkpt) return m.ckpt + (long)idx * st;
   if (buf == kBufSeg) return m.seg + (long)idx * st;
@@ -71,9 +86,8 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
         reduce_intensity_body(m.partial, m.partial_step, L.n_tiles, m.rec_f, m.rec_len, m.sigma_off, m.n, L.n_lines,
                               dE, L.dvol, m.I_out);
     } else if (o.kind == kOpReverse) {
-      dens_pass<N>(m.dens, src, m.rec_d + (long)o.step * m.rec_len, 1.0);
       const double* rec = m.rec_a + (long)o.step * m.rec_len;
-      half_pass<N, 1, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, h});
+      rev_pass<N>(m, src, rec, m.rec_d + (long)o.step * m.rec_len);
       half_pass<N, 0, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, dE});
       half_pass<N, 1, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, h});
     } else if (o.kind == kOpFinal) {

@@ -16,6 +16,10 @@
 // first of step k+1 run as one fused pass: an operation of such a run starts
 // with the first odd half step only if it is the run's first (`first`), and
 // ends with the fused pair if it is not the run's last (`fuse`).
+//
+// A reverse step i runs as the density pullback of step i and the first odd
+// adjoint half step of step i in one fused pass, then the even and the last
+// odd adjoint half steps.
 #pragma once
 
 #include "pnx_kernels.hpp"
@@ -37,8 +41,8 @@ struct MarchOp {
   int src_buf, src_idx;
   int dst_buf, dst_idx;
   int dst2_buf, dst2_idx;  // fused pair: output of the next step's first odd half step
-  int first;      // run the step's first odd half step (first step of a run)
-  int fuse;       // end with the fused pair instead of the last odd half step
+  int first;      // forward: run the step's first odd half step (first step of a run)
+  int fuse;       // forward: end with the fused pair instead of the last odd half step
   int reduce;     // forward finished after this step: form the intensities
 };
 

@@ -49,9 +49,11 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 constexpr size_t kAlign = 256;
-// step slots: fwd odd, fwd even, fwd odd, dens, adj odd, adj even, adj odd, fused fwd odd pair
-constexpr int kSlots = 8;
+// step slots: fwd odd, fwd even, fwd odd, dens, (unused), adj even, adj odd,
+// fused fwd odd pair, fused dens + first adj odd
+constexpr int kSlots = 9;
 constexpr int kSlotPair = 7;
+constexpr int kSlotRev = 8;
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 constexpr int kLanes = 3;      // 1D; 2D/3D of orders <= 5; 2D/3D of orders 7, 9
@@ -455,6 +457,7 @@ double* buffer_of(const Request& r, int buf, int idx) {
 struct SlotWork {
   std::vector<HalfStepArgs> half;
   std::vector<DensArgs> dens;
+  std::vector<RevArgs> rev;
   std::vector<int2> items;
 };
 
@@ -482,6 +485,11 @@ void add_dens(SlotWork& w, const DensArgs& a) {
   const int idx = (int)w.dens.size();
   w.dens.push_back(a);
   for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
+}
+void add_rev(SlotWork& w, const RevArgs& a) {
+  const int idx = (int)w.rev.size();
+  w.rev.push_back(a);
+  for (int t = 0; t < a.adj.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
 }
 
 // Plan the next operation of request r on a grid lane.
@@ -516,9 +524,11 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       break;
     }
     case kOpReverse: {
-      add_dens(w[3], make_dens(s, r, o.step, src, 1.0));
       const double* rec = r.rec_a + (long)o.step * g.rec_len;
-      add_half(w[4], make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr));
+      RevArgs ra{};
+      ra.adj = make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr);
+      ra.dens = make_dens(s, r, o.step, src, 1.0);
+      add_rev(w[kSlotRev], ra);
       add_half(w[5], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr));
       add_half(w[6], make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr));
       break;
@@ -597,7 +607,9 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
         e.n_off = off;
         off = align_up(off + sizeof(int));
         e.args_off = off;
-        off = align_up(off + (sl == 3 ? w.dens.size() * sizeof(DensArgs) : w.half.size() * sizeof(HalfStepArgs)));
+        const size_t ab = sl == 3 ? w.dens.size() * sizeof(DensArgs)
+                                  : (sl >= kSlotRev ? w.rev.size() * sizeof(RevArgs) : w.half.size() * sizeof(HalfStepArgs));
+        off = align_up(off + ab);
         e.items_off = off;
         off = align_up(off + w.items.size() * sizeof(int2));
         entries.push_back(e);
@@ -615,6 +627,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
     if (e.slot == 3)
       std::memcpy(h + e.args_off, w.dens.data(), w.dens.size() * sizeof(DensArgs));
+    else if (e.slot >= kSlotRev)
+      std::memcpy(h + e.args_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
     else
       std::memcpy(h + e.args_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
     std::memcpy(h + e.items_off, w.items.data(), w.items.size() * sizeof(int2));
@@ -635,6 +649,9 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       if (slot == 3) {
         err = launch_dens_grouped(e.key.N, e.key.dims, reinterpret_cast<const DensArgs*>(ln.dev_args + e.args_off),
                                   items, nptr, grid, st);
+      } else if (slot >= kSlotRev) {
+        err = launch_rev_grouped(e.key.N, e.key.dims, reinterpret_cast<const RevArgs*>(ln.dev_args + e.args_off),
+                                 items, nptr, grid, st);
       } else if (slot == kSlotPair) {
         err = launch_half_step2_grouped(e.key.N, e.key.dims, kForward,
                                         reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
@@ -666,7 +683,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
           hipSuccess)
         return -1;
     }
-    for (int sl : {3, 4, 5, 6})
+    for (int sl : {3, kSlotRev, 5, 6})
       if (launch_slot(t, sl)) return -1;
   }
   for (Request* r : plan.copy_out) {
