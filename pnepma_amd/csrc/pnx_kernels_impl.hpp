@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pnx_kernels.hpp"
 #include "pnx_tables.hpp"
 
@@ -592,6 +594,91 @@ __device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
   });
 }
 
+// 2D form with one class per wavefront: a block takes two rows (sub-tile
+// `sub` of 4) of a 32 x 8 tile, and wavefront w the class 2w (eee, eoe, oee,
+// ooe) at those 64 cells.  Each wavefront runs one class's code, so the
+// kernel needs the registers of one class instead of all four, which raises
+// the occupancy of these register-heavy passes; a wavefront still reads two
+// row segments of 32 doubles per moment.
+template <int N, class F>
+__device__ __forceinline__ void class_per_wave_2d(const Layout& L, int tile, int sub, F&& f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tx = tile % L.tiles_x, ty = (tile / L.tiles_x) % L.tiles_y;
+  const int i = tx * tile_x(2) + (lane & 31);
+  const int j = ty * (kBlock / tile_x(2)) + sub * 2 + (lane >> 5);
+  if (i >= L.Mx || j >= L.My) return;
+  const long p = i + (long)L.X * j;
+  static_for<0, 4>([&](auto Wc) {
+    constexpr int C = 2 * decltype(Wc)::value;
+    if constexpr (ncls<N, C>() > 0) {
+      if (wave == decltype(Wc)::value) {
+        const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0));
+        if (interior) f(std::integral_constant<int, C>{}, i, j, p);
+      }
+    }
+  });
+}
+
+template <int N>
+__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int tile, int sub) {
+  const Layout& L = a.adj.L;
+  const long bo = (long)beam * L.beam_stride;
+  gcd lam_b = glb(a.adj.from) + bo;
+  gcd u_b = glb(a.dens.u) + bo;
+  class_per_wave_2d<N>(L, tile, sub, [&](auto Cc, int i, int j, long p) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (parity_of(C) == 1)
+      rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
+    else
+      dens_class<N, 2, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam);
+  });
+}
+
+template <int N>
+__device__ __forceinline__ void dens_body_cw(const DensArgs& a, const double* u, const double* rec, double int_fac,
+                                             int beam, int tile, int sub) {
+  const long bo = (long)beam * a.L.beam_stride;
+  class_per_wave_2d<N>(a.L, tile, sub, [&](auto Cc, int i, int j, long p) {
+    constexpr int C = decltype(Cc)::value;
+    dens_class<N, 2, C>(a, glb(rec), int_fac, glb(u) + bo, glb(a.lam) + bo, i, j, 0, p, beam);
+  });
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void rev_kernel_cw(RevArgs a) {
+  int beam, t4;
+  block_coords(a.adj.L.n_tiles * 4, a.adj.n_beams, beam, t4);
+  rev_body_cw<N>(a, beam, t4 >> 2, t4 & 3);
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void rev_grouped_cw(const RevArgs* __restrict__ args,
+                                                         const int2* __restrict__ items,
+                                                         const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if ((it >> 2) >= *n_items) return;
+  const int2 w = items[it >> 2];
+  rev_body_cw<N>(args[w.x], 0, w.y, it & 3);
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void dens_kernel_cw(DensArgs a) {
+  int beam, t4;
+  block_coords(a.L.n_tiles * 4, a.n_beams, beam, t4);
+  dens_body_cw<N>(a, a.u, a.rec, a.int_fac, beam, t4 >> 2, t4 & 3);
+}
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void dens_grouped_cw(const DensArgs* __restrict__ args,
+                                                          const int2* __restrict__ items,
+                                                          const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if ((it >> 2) >= *n_items) return;
+  const int2 w = items[it >> 2];
+  const DensArgs& a = args[w.x];
+  dens_body_cw<N>(a, a.u, a.rec, a.int_fac, 0, w.y, it & 3);
+}
+
 template <int N, int DIM>
 __global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs a) {
   int beam, tile;
@@ -901,7 +988,7 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
   const dim3 grid(a.adj.L.n_tiles * a.adj.n_beams), block(kBlock);
   switch (dims) {
     case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((rev_kernel<N, 2>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(grid.x * 4), block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((rev_kernel<N, 3>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
@@ -913,7 +1000,7 @@ hipError_t rev_grouped_order(int dims, const RevArgs* args, const int2* items, c
                              hipStream_t s) {
   switch (dims) {
     case 1: hipLaunchKernelGGL((rev_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 2: hipLaunchKernelGGL((rev_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 2: hipLaunchKernelGGL((rev_grouped_cw<N>), dim3(grid * 4), dim3(kBlock), 0, s, args, items, n_items); break;
     case 3: hipLaunchKernelGGL((rev_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
@@ -925,7 +1012,7 @@ hipError_t dens_grouped_order(int dims, const DensArgs* args, const int2* items,
                               hipStream_t s) {
   switch (dims) {
     case 1: hipLaunchKernelGGL((dens_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 2: hipLaunchKernelGGL((dens_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 2: hipLaunchKernelGGL((dens_grouped_cw<N>), dim3(grid * 4), dim3(kBlock), 0, s, args, items, n_items); break;
     case 3: hipLaunchKernelGGL((dens_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
@@ -937,7 +1024,7 @@ hipError_t dens_order(int dims, const DensArgs& a, hipStream_t s) {
   const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
   switch (dims) {
     case 1: hipLaunchKernelGGL((dens_kernel<N, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((dens_kernel<N, 2>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((dens_kernel_cw<N>), dim3(grid.x * 4), block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((dens_kernel<N, 3>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
