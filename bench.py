@@ -27,6 +27,14 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The server gives each group of PN orders its own stream; the runtime maps
+# streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by default) and
+# serialises streams that share one.  Six queues leave one each for the default
+# stream, the context stream (1D lane) and four 2D/3D lanes.  Read by the HIP
+# runtime when it initialises, so it is set before anything touches the GPU
+# (raised to 6 when the environment asks for fewer, as the default 4 does).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 6:
+    os.environ["GPU_MAX_HW_QUEUES"] = "6"
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "served fwd+adj solves/s, p50/p99 latency vs plain GPU loop @ k-ratio err<=1e-10"

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -56,9 +57,31 @@ constexpr int kSlotPair = 7;
 constexpr int kSlotRev = 8;
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
-constexpr int kLanes = 3;      // 1D; 2D/3D of orders <= 5; 2D/3D of orders 7, 9
+// 1D, and up to four 2D/3D lanes.  (Six lanes, with orders 7 and 9 split by
+// grid size, measured slower than four: the grouped launches get smaller.)
+constexpr int kMaxLanes = 5;
 
-int lane_of(int N, int dims) { return dims == 1 ? 0 : (N <= 5 ? 1 : 2); }
+// Lane of a request.  With two 2D/3D lanes: orders <= 5 and orders 7, 9; with
+// three: 1-3, 5-7, 9; with four: 1-3, 5, 7, 9.
+int lane_of(int N, int dims, int grid_lanes) {
+  if (dims == 1) return 0;
+  switch (grid_lanes) {
+    case 1: return 1;
+    case 2: return N <= 5 ? 1 : 2;
+    case 3: return N <= 3 ? 1 : (N <= 7 ? 2 : 3);
+    default: return N <= 3 ? 1 : (N == 5 ? 2 : (N == 7 ? 3 : 4));
+  }
+}
+
+// 2D/3D lanes: one stream each, besides the context stream (1D lane) and the
+// default stream, so that every stream has a hardware queue of its own (the
+// runtime gives a process GPU_MAX_HW_QUEUES queues, 4 by default; streams
+// beyond that share queues and their kernels serialise).
+int grid_lanes_for_queues() {
+  int q = 4;
+  if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::atoi(e);
+  return std::max(1, std::min(kMaxLanes - 1, q - 2));
+}
 
 size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
 // First-fit allocator of offsets within a range of `size` bytes.
@@ -301,7 +324,8 @@ struct pnx_server {
   int64_t next_ticket = 1;
   std::atomic<bool> stop{false};
   std::thread worker;
-  Lane lanes[kLanes];
+  Lane lanes[kMaxLanes];
+  int n_lanes = 3;
   PinnedPool pinned_pool;
   // coefficient buffers per order and direction
   std::map<int, double*> coef_f, coef_a;
@@ -814,7 +838,7 @@ void worker_loop(pnx_server* s) {
     }
     bool launched = false;
     std::vector<std::shared_ptr<Request>> finished;
-    for (int li = 0; li < kLanes; ++li) {
+    for (int li = 0; li < s->n_lanes; ++li) {
       Lane& ln = s->lanes[li];
       if (ln.active.empty()) continue;
       if (poll_lane(ln)) {
@@ -847,7 +871,8 @@ void worker_loop(pnx_server* s) {
     }
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      s->ticks = s->lanes[0].tick + s->lanes[1].tick + s->lanes[2].tick;
+      s->ticks = 0;
+      for (int li = 0; li < s->n_lanes; ++li) s->ticks += s->lanes[li].tick;
     }
     if (!launched && finished.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
@@ -899,10 +924,10 @@ int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_ac
     release_server(s);
     return set_error(PNX_ERR_DEVICE, "coefficient upload failed");
   }
-  // The 1D lane runs on the context's stream: with the runtime's default of
-  // four hardware queues per process, two more streams leave every lane a
-  // queue of its own next to the default stream.
-  for (int li = 0; li < kLanes; ++li) {
+  // The 1D lane runs on the context's stream, the 2D/3D lanes on streams of
+  // their own (grid_lanes_for_queues).
+  s->n_lanes = 1 + grid_lanes_for_queues();
+  for (int li = 0; li < s->n_lanes; ++li) {
     Lane& ln = s->lanes[li];
     if (li == 0) {
       ln.stream = ctx->stream;
@@ -953,7 +978,7 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   int rc = make_geometry(r->d, r->g, err);
   if (rc) return set_error(rc, err);
   r->sz = request_sizes(r->g, r->d);
-  r->lane = lane_of(r->d.N, r->g.L.dims);
+  r->lane = lane_of(r->d.N, r->g.L.dims, s->n_lanes - 1);
   r->off_ = request_offsets(r->sz);
   const Offsets& o = r->off_;
   r->bytes = (size_t)o.total * sizeof(double);
