@@ -82,6 +82,24 @@ int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj
  * line's element (x_ray_intensities.jl:183). */
 int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const double* W);
 
+/* Material inputs from per-cell mass fractions, with the densities computed on
+ * the device.  Replaces the host side of component_densities!
+ * (pn_solver/pn_parametrizations/pn_material.jl:233-264) for a MassFraction
+ * material whose box parametrization (pn_parametrization_boxed.jl:137-175)
+ * has one box per grid cell: boxes centred on the even points, edges at the
+ * odd points.  w is [cell][element] with cells x fastest (Mx*My*Mz cells),
+ * rho_pure[element] the pure-element densities, W as pnx_problem_set_material. */
+int pnx_problem_set_cell_fractions(pnx_problem* p, const double* w, const double* rho_pure, const double* W);
+
+/* Gradient of sum(rho_bar * rho) with respect to the cell fractions, through
+ * the same map (component_densities_pullback!, pn_material.jl, and the box
+ * parametrization's pullback).  rho_bar is the density gradient of the last
+ * pnx_forward_adjoint, summed over beams; extra_eee ([eee point][element],
+ * reference layout, may be NULL) is added at eee points (the number-of-atoms
+ * term of pn_intensities.jl:406-410).  grad is [cell][element].  Requires
+ * pnx_problem_set_cell_fractions. */
+int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, double* grad);
+
 /* Beam source.  mms[c] = beam moments of each x-odd class c (n_c values),
  * ppos[b*8 + c] = position pdf over the (y, z) points of class c for beam b
  * (y fastest); NULL for x-even classes (pn_problem.jl:192-203). */

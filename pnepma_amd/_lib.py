@@ -45,9 +45,11 @@ SIGNATURES = {
     "pnx_problem_set_records": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pnx_problem_set_material": (C.c_int, [C.c_void_p, _dp * 8, _dp]),
     "pnx_problem_set_beams": (C.c_int, [C.c_void_p, _dp * 8, C.POINTER(_dp)]),
+    "pnx_problem_set_cell_fractions": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "pnx_problem_cell_fraction_gradient": (C.c_int, [C.c_void_p, _dp, _dp]),
     "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
-    "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, _dp * 8, _dp]),
-    "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, _dp * 8, _dp]),
+    "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
+    "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
     "pnx_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp * 8]),
     "pnx_forward_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pnx_forward_adjoint_timed": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

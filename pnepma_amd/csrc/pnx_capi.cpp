@@ -75,6 +75,8 @@ struct pnx_problem {
   bool have_rec = false, have_adj = false, have_mat = false, have_beams = false;
   DevBuf state, rho, W, coef_f, coef_a, rec_f, rec_a, rec_d, prof, mms, partial, I_dev;
   DevBuf lam, ckpt, seg, rho_bar_b, gen_b, ibar, sum_buf, w1, w2;
+  DevBuf cell_w, rho_pure, extra, grad;  // device material layer
+  bool have_cells = false, have_rho_bar = false;
   hipGraphExec_t fwd_graph = nullptr;
 };
 
@@ -249,6 +251,56 @@ int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const d
   HIP_TRY(hipMemcpyAsync(p->W.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
   p->have_mat = true;
+  return PNX_OK;
+}
+
+int pnx_problem_set_cell_fractions(pnx_problem* p, const double* w, const double* rho_pure, const double* W) {
+  if (!p || !w || !rho_pure || !W) return fail(PNX_ERR_ARG, "null argument");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const Geometry& g = p->g;
+  const int ne = p->d.n_elements;
+  for (int e = 0; e < ne; ++e)
+    if (!(rho_pure[e] > 0.0)) return fail(PNX_ERR_ARG, "pure-element densities must be positive");
+  const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
+  int rc;
+  if ((rc = p->cell_w.alloc(n_cells * ne)) || (rc = p->rho_pure.alloc(ne))) return rc;
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->cell_w.p, w, n_cells * ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->rho_pure.p, rho_pure, ne * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_fill(p->rho.p, g.rho_total, 0.0, s));
+  HIP_TRY(launch_material_densities(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho.p, s));
+  std::vector<double> wl((size_t)p->d.n_lines * g.L.V, 0.0);
+  pack_lines(g, W, wl.data());
+  HIP_TRY(hipMemcpyAsync(p->W.p, wl.data(), wl.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  destroy_graph(p);
+  p->have_mat = true;
+  p->have_cells = true;
+  return PNX_OK;
+}
+
+int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, double* grad) {
+  if (!p || !grad) return fail(PNX_ERR_ARG, "null argument");
+  if (!p->have_cells) return fail(PNX_ERR_STATE, "the material was not set from cell fractions");
+  if (!p->have_rho_bar) return fail(PNX_ERR_STATE, "no density gradient yet: run pnx_forward_adjoint first");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const Geometry& g = p->g;
+  const int ne = p->d.n_elements;
+  const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
+  hipStream_t s = p->ctx->stream;
+  int rc;
+  if ((rc = p->grad.alloc(n_cells * ne))) return rc;
+  std::vector<double> h;
+  if (extra_eee) {
+    if ((rc = p->extra.alloc((size_t)ne * g.L.V))) return rc;
+    h.assign((size_t)ne * g.L.V, 0.0);
+    pack_class(g, 0, extra_eee, ne, h.data());
+    HIP_TRY(hipMemcpyAsync(p->extra.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(launch_material_pullback(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho_bar_b.p,
+                                   p->rb_stride, p->d.n_beams, extra_eee ? p->extra.p : nullptr, p->grad.p, s));
+  HIP_TRY(hipMemcpyAsync(grad, p->grad.p, n_cells * ne * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;
 }
 
@@ -596,6 +648,7 @@ static int forward_adjoint_impl(pnx_problem* p, bool exact, const double* I_bar,
   hipStream_t s = p->ctx->stream;
   HIP_TRY(hipMemcpyAsync(p->ibar.p, I_bar, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyHostToDevice, s));
   if ((rc = exact ? enqueue_forward_adjoint_exact(p) : enqueue_forward_adjoint(p))) return rc;
+  p->have_rho_bar = true;
   HIP_TRY(hipMemcpyAsync(I_out, p->I_dev.p, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
   const Layout& L = p->L;
   const int ne = p->d.n_elements;

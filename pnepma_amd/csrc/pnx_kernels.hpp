@@ -147,6 +147,16 @@ hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* 
                               hipStream_t s);
 // pb, pa in {-1 (none), 0 (even), 1 (odd)}
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+// Device material layer (pnx_material.hip): densities of every active class
+// from per-cell mass fractions w[cell][e], and the pullback of the per-beam
+// density gradient (plus an optional eee term, [e][V]) to grad[cell][e].
+hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], const long rho_off[8],
+                                     const int ncls_of[8], const double* w, const double* rho_pure, double* rho,
+                                     hipStream_t s);
+hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], const long rho_off[8],
+                                    const int ncls_of[8], const double* w, const double* rho_pure,
+                                    const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
+                                    double* grad, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
 hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);

@@ -16,6 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import Desc, check, dptr, lib
+from .material import CellFractions, MassFractionMaterial
 from .problem import ForwardPNProblem, SolveInputs, absorption_factors, absorption_pullback, solve_inputs
 
 
@@ -101,8 +102,32 @@ class DeviceProblem:
 
     def set_material(self, inputs: SolveInputs):
         self.inputs = inputs
+        self.cells = None
         r8 = (_lib._dp * 8)(*[dptr(np.ascontiguousarray(r)) for r in inputs.rho])
         check(lib().pnx_problem_set_material(self.handle, r8, dptr(np.ascontiguousarray(inputs.W))))
+
+    def set_cell_fractions(self, material: MassFractionMaterial, inputs: SolveInputs):
+        """Densities of every class computed on the device from the per-cell
+        fractions of a grid-aligned box material (see cell_aligned)."""
+        if not cell_aligned(material, self.problem.gspec):
+            raise ValueError("the material's boxes are not the grid's cells")
+        self.inputs = inputs
+        vals = material.param.vals                        # (nx, ny, nz, n_e)
+        self.cells = vals.shape
+        w = np.ascontiguousarray(vals.transpose(2, 1, 0, 3).reshape(-1, vals.shape[3]))  # x fastest
+        rho_pure = np.ascontiguousarray(material.rho_pure, dtype=np.float64)
+        check(lib().pnx_problem_set_cell_fractions(self.handle, dptr(w), dptr(rho_pure),
+                                                   dptr(np.ascontiguousarray(inputs.W))))
+
+    def cell_fraction_gradient(self, extra_eee=None) -> np.ndarray:
+        """d(rho_bar . rho)/d(cell fractions) on the device, flattened like
+        CellFractions.params (x slowest, element fastest); rho_bar is the last
+        forward_adjoint's, plus extra_eee ([eee point][element]) at eee points."""
+        nx, ny, nz, ne = self.cells
+        g = np.zeros((nx * ny * nz, ne))
+        ex = None if extra_eee is None else np.ascontiguousarray(extra_eee, dtype=np.float64)
+        check(lib().pnx_problem_cell_fraction_gradient(self.handle, dptr(ex), dptr(g)))
+        return g.reshape(nz, ny, nx, ne).transpose(2, 1, 0, 3).ravel()
 
     def close(self):
         if getattr(self, "handle", None):
@@ -124,19 +149,20 @@ class DeviceProblem:
         check(lib().pnx_forward(self.handle, dptr(out)))
         return out
 
-    def forward_adjoint(self, I_bar, exact: bool = False):
+    def forward_adjoint(self, I_bar, exact: bool = False, want_rho_bar: bool = True):
         """Returns (I, rho_bar[8] per class (P_c, n_e) summed over beams, gen (P_eee, n_lines)).
 
         exact=False runs the reference's adjoint; exact=True the exact gradient of the
-        discrete forward march."""
+        discrete forward march.  want_rho_bar=False leaves rho_bar on the device
+        (for cell_fraction_gradient) and returns None in its place."""
         p = self.problem
         ib = np.ascontiguousarray(np.broadcast_to(np.asarray(I_bar, dtype=np.float64),
                                                   (p.n_beams, len(p.x_rays))))
         I = np.zeros((p.n_beams, len(p.x_rays)))
         ne = len(p.elements)
-        rb = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)]
+        rb = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)] if want_rho_bar else None
         gen = np.zeros((int(np.prod(p.gspec.class_size(0))), len(p.x_rays)))
-        r8 = (_lib._dp * 8)(*[dptr(r) for r in rb])
+        r8 = (_lib._dp * 8)(*[dptr(r) for r in rb]) if want_rho_bar else None
         fn = lib().pnx_forward_adjoint_exact if exact else lib().pnx_forward_adjoint
         check(fn(self.handle, dptr(ib), dptr(I), r8, dptr(gen)))
         return I, rb, gen
@@ -173,21 +199,42 @@ def calc_intensities(problem: ForwardPNProblem, material, detector, ctx: Context
         dp.close()
 
 
-def material_gradient(problem: ForwardPNProblem, material, detector, si: SolveInputs, rho_bar, gen) -> np.ndarray:
+def cell_aligned(material, gspec) -> bool:
+    """A MassFraction material with one box per grid cell (boxes centred on the
+    even points, edges at the odd points): the case the device material layer
+    handles."""
+    from .workloads import cell_edges
+    if not isinstance(material, MassFractionMaterial) or not isinstance(material.param, CellFractions):
+        return False
+    par = material.param
+    if par.vals.shape[:3] != tuple(a.M for a in gspec.axes):
+        return False
+    return all(np.array_equal(e, cell_edges(a)) for e, a in zip(par.edges, gspec.axes))
+
+
+def material_gradient(problem: ForwardPNProblem, material, detector, si: SolveInputs, rho_bar, gen,
+                      device: "DeviceProblem | None" = None) -> np.ndarray:
     """Material pullbacks of the reference adjoint (pn_intensities.jl:406-410):
-    densities at every class point, absorption factors and numbers of atoms."""
+    densities at every class point, absorption factors and numbers of atoms.
+
+    With `device` (a DeviceProblem whose material was set by set_cell_fractions
+    and which has run forward_adjoint), the density and number-of-atoms terms
+    are pulled back on the device and rho_bar is not needed on the host."""
     g = problem.gspec
-    grad = np.zeros(material.param.n_params)
-    for c in range(8):
-        if np.any(rho_bar[c]):
-            grad += material.densities_pullback(g.class_points(c), rho_bar[c])
     ac_bar = si.noa[:, si.el_idx] * gen
     noa_bar = np.zeros_like(si.noa)
     np.add.at(noa_bar.T, si.el_idx, (si.ac * gen).T)
+    am = np.array([e.atom_mass_g for e in material.elements])
+    if device is not None:
+        grad = device.cell_fraction_gradient(extra_eee=noa_bar / am)
+    else:
+        grad = np.zeros(material.param.n_params)
+        for c in range(8):
+            if np.any(rho_bar[c]):
+                grad += material.densities_pullback(g.class_points(c), rho_bar[c])
+        grad += material.densities_pullback(g.class_points(0), noa_bar / am)
     ac, aux = absorption_factors(material, problem.x_rays, g, detector, problem.beam_positions[0][0])
     grad += absorption_pullback(material, aux, ac, ac_bar)
-    am = np.array([e.atom_mass_g for e in material.elements])
-    grad += material.densities_pullback(g.class_points(0), noa_bar / am)
     return grad
 
 
@@ -199,11 +246,14 @@ def calc_intensities_pullback(problem: ForwardPNProblem, material, detector, ckp
     march with checkpoints (ckpt_every energy steps) on the device."""
     si = solve_inputs(problem, material, detector)
     dp = DeviceProblem(problem, si, ctx, ckpt_every=ckpt_every)
+    on_device = cell_aligned(material, problem.gspec)
+    if on_device:
+        dp.set_cell_fractions(material, si)
     I = dp.forward()
 
     def pullback(I_bar):
-        _, rb, gen = dp.forward_adjoint(I_bar)
-        return material_gradient(problem, material, detector, si, rb, gen)
+        _, rb, gen = dp.forward_adjoint(I_bar, want_rho_bar=not on_device)
+        return material_gradient(problem, material, detector, si, rb, gen, device=dp if on_device else None)
 
     pullback.device_problem = dp
     return I, pullback

@@ -1,0 +1,92 @@
+"""Device material layer: class densities from per-cell mass fractions, and the
+pullback of the density gradient to those fractions, against the host
+restatement (pnepma_amd.material, pn_material.jl / pn_parametrization_boxed.jl).
+
+* densities: a problem whose densities were computed on the device gives the
+  same intensities as one given the host densities (bitwise: the two compute
+  w_e / sum(w / rho_pure) in the same order);
+* pullback: the device gradient of sum(rho_bar * rho) (plus the eee
+  number-of-atoms term) equals the host densities_pullback summed over the
+  classes, to 1e-12;
+* calc_intensities_pullback takes the device path for grid-aligned box
+  materials and agrees with the host path to 1e-12."""
+import numpy as np
+import pytest
+
+from pnepma_amd import solver, workloads
+from pnepma_amd.grid import GridSpec
+from pnepma_amd.material import CellFractions, MassFractionMaterial
+from pnepma_amd.physics import XRay, element
+from pnepma_amd.problem import ForwardPNProblem, solve_inputs
+from pnepma_amd.solver import DeviceProblem, cell_aligned
+
+from .helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(M, N=3, n_steps=30, seed=4, n_beams=2):
+    rng = np.random.default_rng(seed)
+    names = ["Fe", "Cr", "Ni"]
+    els = [element(s) for s in names]
+    dims = sum(m > 1 for m in M)
+    lo = [(-1.5e-4, 0.0), (-0.5e-4, 0.5e-4), (-0.4e-4, 0.4e-4)]
+    g = GridSpec.make(M, lo[0], lo[1] if dims >= 2 else 0.0, lo[2] if dims >= 3 else 0.0)
+    ys = np.linspace(-0.3e-4, 0.3e-4, n_beams) if dims >= 2 else np.zeros(n_beams)
+    p = ForwardPNProblem(N=N, gspec=g, elements=els, beam_energy=(15e3, workloads.SIGMA_E),
+                         beam_positions=[(0.0, float(y), 0.0) for y in ys], beam_sigma=50 * workloads.NM,
+                         beam_direction=((-1.0, 0.0, 0.0), workloads.KAPPA), E_initial=15.5e3, E_cutoff=5e3,
+                         n_steps=n_steps, source_mode="lateral")
+    p.x_rays = [XRay(s) for s in names]
+    vals = workloads.inclusion_fractions(tuple(M), len(els), 4, rng)
+    mat = MassFractionMaterial(els, CellFractions(vals, *[workloads.cell_edges(a) for a in g.axes],
+                                                  embedding=vals.reshape(-1, len(els))[0]))
+    return p, mat
+
+
+@pytest.mark.parametrize("M", [(40, 1, 1), (20, 14, 1), (10, 8, 6)], ids=["1d", "2d", "3d"])
+def test_device_densities_give_the_same_intensities(M):
+    p, mat = _problem(M)
+    assert cell_aligned(mat, p.gspec)
+    si = solve_inputs(p, mat, workloads.detector_40deg())
+    I_host = DeviceProblem(p, si, adjoint=False).forward()
+    dp = DeviceProblem(p, si, adjoint=False)
+    dp.set_cell_fractions(mat, si)
+    I_dev = dp.forward()
+    print("intensities host", I_host[0], "device", I_dev[0], flush=True)
+    assert np.array_equal(I_host, I_dev)
+
+
+@pytest.mark.parametrize("M", [(40, 1, 1), (20, 14, 1), (10, 8, 6)], ids=["1d", "2d", "3d"])
+def test_device_fraction_gradient_matches_host(M):
+    p, mat = _problem(M)
+    si = solve_inputs(p, mat, workloads.detector_40deg())
+    dp = DeviceProblem(p, si, ckpt_every=8)
+    dp.set_cell_fractions(mat, si)
+    ib = np.random.default_rng(5).uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
+    _, rb, gen = dp.forward_adjoint(ib)
+    extra = np.random.default_rng(6).normal(size=rb[0].shape)
+    g_dev = dp.cell_fraction_gradient(extra_eee=extra)
+    g_host = np.zeros(mat.param.n_params)
+    for c in range(8):
+        if np.any(rb[c]):
+            g_host += mat.densities_pullback(p.gspec.class_points(c), rb[c])
+    g_host += mat.densities_pullback(p.gspec.class_points(0), extra)
+    e = rel_err(g_dev, g_host)
+    print(f"cell-fraction gradient rel err {e:.3e}", flush=True)
+    assert e < 1e-12
+
+
+def test_calc_intensities_pullback_device_path():
+    p, mat = _problem((20, 14, 1), n_beams=3)
+    det = workloads.detector_40deg()
+    ib = np.random.default_rng(7).uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
+    I, back = solver.calc_intensities_pullback(p, mat, det, ckpt_every=8)
+    g_dev = back(ib)
+    si = solve_inputs(p, mat, det)
+    dp = DeviceProblem(p, si, ckpt_every=8)
+    _, rb, gen = dp.forward_adjoint(ib)
+    g_host = solver.material_gradient(p, mat, det, si, rb, gen)
+    e = rel_err(g_dev, g_host)
+    print(f"material gradient device vs host rel err {e:.3e}", flush=True)
+    assert e < 1e-12
