@@ -100,6 +100,30 @@ int pnx_problem_set_cell_fractions(pnx_problem* p, const double* w, const double
  * pnx_problem_set_cell_fractions. */
 int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, double* grad);
 
+/* Absorption factors on the device (compute_absorption_coefficients,
+ * pn_solver/x_ray_intensities.jl:100-128): from every eee point a 100-sample
+ * march towards `detector`, cut at the plane x = beam_x0, through the boxes of
+ * pnx_problem_set_cell_fractions (the embedding fractions outside them).  Also
+ * sets the line weights W = ac * number of atoms of the line's element.
+ * mac[line][element] mass absorption coefficients, atom_mass[element] (g),
+ * line_element[line] element index of each line, axis_points[a] the M_a even
+ * points of axis a, cell_edges[a] its M_a + 1 box edges (a singleton axis:
+ * one box, two edges).  ac_out ([eee point][line], may be NULL) receives the
+ * factors.  Requires pnx_problem_set_cell_fractions. */
+int pnx_problem_set_absorption(pnx_problem* p, const double* mac, const double* atom_mass,
+                               const int32_t* line_element, const double* detector, double beam_x0,
+                               const double* const axis_points[3], const double* const cell_edges[3],
+                               const double* embedding, double* ac_out);
+
+/* The whole gradient of sum_{b,r} I_bar I with respect to the cell fractions
+ * after pnx_forward_adjoint (the material pullbacks of pn_intensities.jl:406-410:
+ * densities, number of atoms and absorption factors,
+ * x_ray_intensities.jl:131-153, with its fixed-factor approximation), on the
+ * device.  grad is [cell][element], cells x fastest.  The absorption term is
+ * accumulated with atomic additions, so its rounding depends on their order.
+ * Requires pnx_problem_set_cell_fractions and pnx_problem_set_absorption. */
+int pnx_problem_material_gradient(pnx_problem* p, double* grad);
+
 /* Beam source.  mms[c] = beam moments of each x-odd class c (n_c values),
  * ppos[b*8 + c] = position pdf over the (y, z) points of class c for beam b
  * (y fastest); NULL for x-even classes (pn_problem.jl:192-203). */

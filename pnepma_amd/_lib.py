@@ -47,6 +47,9 @@ SIGNATURES = {
     "pnx_problem_set_beams": (C.c_int, [C.c_void_p, _dp * 8, C.POINTER(_dp)]),
     "pnx_problem_set_cell_fractions": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "pnx_problem_cell_fraction_gradient": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "pnx_problem_set_absorption": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_double,
+                                             _dp * 3, _dp * 3, _dp, _dp]),
+    "pnx_problem_material_gradient": (C.c_int, [C.c_void_p, _dp]),
     "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
     "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
     "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),

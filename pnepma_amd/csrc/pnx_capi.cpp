@@ -76,7 +76,11 @@ struct pnx_problem {
   DevBuf state, rho, W, coef_f, coef_a, rec_f, rec_a, rec_d, prof, mms, partial, I_dev;
   DevBuf lam, ckpt, seg, rho_bar_b, gen_b, ibar, sum_buf, w1, w2;
   DevBuf cell_w, rho_pure, extra, grad;  // device material layer
-  bool have_cells = false, have_rho_bar = false;
+  DevBuf abs_small, mu, ac;             // device absorption: points/edges/mac/masses/embedding, table, factors
+  int* line_el = nullptr;
+  AbsorptionGeometry abs{};
+  long abs_off_mac = 0, abs_off_am = 0, abs_off_emb = 0;
+  bool have_cells = false, have_abs = false, have_rho_bar = false;
   hipGraphExec_t fwd_graph = nullptr;
 };
 
@@ -201,6 +205,7 @@ int pnx_problem_destroy(pnx_problem* p) {
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
   destroy_graph(p);
+  if (p->line_el) (void)hipFree(p->line_el);
   delete p;
   return PNX_OK;
 }
@@ -299,6 +304,101 @@ int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, 
   }
   HIP_TRY(launch_material_pullback(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho_bar_b.p,
                                    p->rb_stride, p->d.n_beams, extra_eee ? p->extra.p : nullptr, p->grad.p, s));
+  HIP_TRY(hipMemcpyAsync(grad, p->grad.p, n_cells * ne * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return PNX_OK;
+}
+
+int pnx_problem_set_absorption(pnx_problem* p, const double* mac, const double* atom_mass,
+                               const int32_t* line_element, const double* detector, double beam_x0,
+                               const double* const axis_points[3], const double* const cell_edges[3],
+                               const double* embedding, double* ac_out) {
+  if (!p || !mac || !atom_mass || !line_element || !detector || !axis_points || !cell_edges || !embedding)
+    return fail(PNX_ERR_ARG, "null argument");
+  if (!p->have_cells) return fail(PNX_ERR_STATE, "set the cell fractions first (pnx_problem_set_cell_fractions)");
+  const Geometry& g = p->g;
+  const int ne = p->d.n_elements, nl = p->d.n_lines;
+  const int M[3] = {g.L.Mx, g.L.My, g.L.Mz};
+  for (int r = 0; r < nl; ++r)
+    if (line_element[r] < 0 || line_element[r] >= ne) return fail(PNX_ERR_ARG, "line element index out of range");
+  for (int a = 0; a < 3; ++a)
+    if (!axis_points[a] || !cell_edges[a]) return fail(PNX_ERR_ARG, "null axis points or edges");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  // small inputs in one buffer: points and edges per axis, mac, atom masses, embedding
+  std::vector<double> h;
+  long off_pts[3], off_edges[3];
+  for (int a = 0; a < 3; ++a) {
+    off_pts[a] = (long)h.size();
+    h.insert(h.end(), axis_points[a], axis_points[a] + M[a]);
+    off_edges[a] = (long)h.size();
+    h.insert(h.end(), cell_edges[a], cell_edges[a] + M[a] + 1);  // one box per point (a singleton axis: one)
+  }
+  p->abs_off_mac = (long)h.size();
+  h.insert(h.end(), mac, mac + (size_t)nl * ne);
+  p->abs_off_am = (long)h.size();
+  h.insert(h.end(), atom_mass, atom_mass + ne);
+  p->abs_off_emb = (long)h.size();
+  h.insert(h.end(), embedding, embedding + ne);
+  int rc;
+  const long n_boxes = (long)M[0] * M[1] * M[2];
+  if ((rc = p->abs_small.alloc(h.size())) || (rc = p->mu.alloc((size_t)(n_boxes + 1) * nl)) ||
+      (rc = p->ac.alloc((size_t)nl * g.L.V)))
+    return rc;
+  if (!p->line_el && hipMalloc(&p->line_el, 8 * sizeof(int)) != hipSuccess) {
+    p->line_el = nullptr;
+    return fail(PNX_ERR_MEMORY, "hipMalloc failed");
+  }
+  hipStream_t s = p->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(p->abs_small.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(p->line_el, line_element, nl * sizeof(int), hipMemcpyHostToDevice, s));
+  AbsorptionGeometry& ag = p->abs;
+  for (int a = 0; a < 3; ++a) {
+    ag.points[a] = p->abs_small.p + off_pts[a];
+    ag.edges[a] = p->abs_small.p + off_edges[a];
+    ag.n_boxes[a] = M[a];
+    const double* e = cell_edges[a];
+    ag.inv_span[a] = (double)M[a] / (e[M[a]] - e[0]);
+    ag.detector[a] = detector[a];
+  }
+  ag.beam_x0 = beam_x0;
+  HIP_TRY(launch_fill(p->ac.p, (long)nl * g.L.V, 0.0, s));
+  HIP_TRY(launch_fill(p->W.p, (long)nl * g.L.V, 0.0, s));
+  HIP_TRY(launch_absorption(ag, g.L, p->cell_w.p, p->abs_small.p + p->abs_off_emb, p->rho_pure.p,
+                            p->abs_small.p + p->abs_off_mac, p->rho.p + g.rho_off[0], p->abs_small.p + p->abs_off_am,
+                            p->line_el, p->mu.p, p->ac.p, p->W.p, s));
+  if (ac_out) {
+    std::vector<double> a((size_t)nl * g.L.V);
+    HIP_TRY(hipMemcpyAsync(a.data(), p->ac.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    unpack_lines(g, a.data(), ac_out);
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  destroy_graph(p);
+  p->have_abs = true;
+  return PNX_OK;
+}
+
+int pnx_problem_material_gradient(pnx_problem* p, double* grad) {
+  if (!p || !grad) return fail(PNX_ERR_ARG, "null argument");
+  if (!p->have_cells || !p->have_abs)
+    return fail(PNX_ERR_STATE, "set the cell fractions and the absorption on the device first");
+  if (!p->have_rho_bar) return fail(PNX_ERR_STATE, "no density gradient yet: run pnx_forward_adjoint first");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  const Geometry& g = p->g;
+  const int ne = p->d.n_elements;
+  const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
+  hipStream_t s = p->ctx->stream;
+  int rc;
+  if ((rc = p->grad.alloc(n_cells * ne)) || (rc = p->extra.alloc((size_t)ne * g.L.V))) return rc;
+  const long gs = (long)p->d.n_lines * g.L.V;
+  const double* am = p->abs_small.p + p->abs_off_am;
+  HIP_TRY(launch_fill(p->extra.p, (long)ne * g.L.V, 0.0, s));
+  HIP_TRY(launch_atoms_bar(p->abs, g.L, p->ac.p, p->gen_b.p, gs, p->d.n_beams, am, p->line_el, p->extra.p, s));
+  HIP_TRY(launch_material_pullback(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho_bar_b.p,
+                                   p->rb_stride, p->d.n_beams, p->extra.p, p->grad.p, s));
+  HIP_TRY(launch_absorption_pullback(p->abs, g.L, p->cell_w.p, p->rho_pure.p, p->abs_small.p + p->abs_off_mac, p->ac.p,
+                                     p->rho.p + g.rho_off[0], am, p->line_el, p->gen_b.p, gs, p->d.n_beams, p->grad.p,
+                                     s));
   HIP_TRY(hipMemcpyAsync(grad, p->grad.p, n_cells * ne * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;

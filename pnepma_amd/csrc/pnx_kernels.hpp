@@ -157,6 +157,27 @@ hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], co
                                     const int ncls_of[8], const double* w, const double* rho_pure,
                                     const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
                                     double* grad, hipStream_t s);
+// Absorption factors on the device (pnx_material.hip).  Device pointers:
+// even points and box edges per axis, detector, the plane x = beam_x0 that
+// cuts the rays; inv_span[a] = n_boxes[a] / (last edge - first edge).
+struct AbsorptionGeometry {
+  const double* points[3];
+  const double* edges[3];
+  int n_boxes[3];
+  double inv_span[3];
+  double detector[3];
+  double beam_x0;
+};
+hipError_t launch_absorption(const AbsorptionGeometry& a, const Layout& L, const double* w, const double* emb,
+                             const double* rho_pure, const double* mac, const double* rho, const double* atom_mass,
+                             const int* line_el, double* mu_scratch, double* ac, double* W, hipStream_t s);
+hipError_t launch_atoms_bar(const AbsorptionGeometry& a, const Layout& L, const double* ac, const double* gen_b,
+                            long gen_stride, int n_beams, const double* atom_mass, const int* line_el, double* extra,
+                            hipStream_t s);
+hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* w,
+                                      const double* rho_pure, const double* mac, const double* ac, const double* rho,
+                                      const double* atom_mass, const int* line_el, const double* gen_b,
+                                      long gen_stride, int n_beams, double* grad, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
 hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);

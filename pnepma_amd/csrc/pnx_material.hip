@@ -133,6 +133,213 @@ MatGeom make_mat_geom(const Layout& L, const int class_n[8][3], const long rho_o
 
 bool class_active_dims(int c, int dims) { return (dims >= 2 || ((c >> 1) & 1) == 0) && (dims >= 3 || (c & 1) == 0); }
 
+// ---------------------------------------------------------------- absorption
+//
+// compute_absorption_coefficients (pn_solver/x_ray_intensities.jl:100-128):
+// from every eee point a 100-sample march towards the detector, cut where the
+// ray crosses the plane x = beam_x0; each sample adds the attenuation
+// sum_e mac[r][e] rho_e of the box it falls in (boxes found by arithmetic on
+// the uniform edges; samples outside the boxes use the embedding fractions).
+// The arithmetic follows the host restatement step by step, without
+// contracting products into fused multiply-adds.
+
+struct AbsGeom {
+  const double* pts[3];   // even points per axis (eee point coordinates)
+  const double* edges[3]; // box edges per axis (n[a] + 1 values)
+  int n[3];               // boxes per axis
+  double inv[3];          // n / (last edge - first edge)
+  int M[3];
+  int X, Y;
+  long V;
+  int n_e, n_lines;
+  double det[3];
+  double beam_x0;
+};
+
+AbsGeom abs_geom(const AbsorptionGeometry& a, const Layout& L) {
+  AbsGeom g{};
+  for (int d = 0; d < 3; ++d) {
+    g.pts[d] = a.points[d];
+    g.edges[d] = a.edges[d];
+    g.n[d] = a.n_boxes[d];
+    g.inv[d] = a.inv_span[d];
+    g.det[d] = a.detector[d];
+  }
+  g.M[0] = L.Mx;
+  g.M[1] = L.My;
+  g.M[2] = L.Mz;
+  g.X = L.X;
+  g.Y = L.Y;
+  g.V = L.V;
+  g.n_e = L.n_e;
+  g.n_lines = L.n_lines;
+  g.beam_x0 = a.beam_x0;
+  return g;
+}
+
+// attenuation table: rows = boxes (x fastest), then the embedding row
+__global__ void mu_table_kernel(AbsGeom g, const double* __restrict__ w, const double* __restrict__ emb,
+                                const double* __restrict__ rho_pure, const double* __restrict__ mac,
+                                double* __restrict__ mu) {
+#pragma clang fp contract(off)
+  const long n_rows = (long)g.n[0] * g.n[1] * g.n[2] + 1;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  const double* wr = row == n_rows - 1 ? emb : w + row * g.n_e;
+  double R = 0.0;
+  for (int e = 0; e < g.n_e; ++e) R += wr[e] / rho_pure[e];
+  for (int r = 0; r < g.n_lines; ++r) {
+    double m = 0.0;
+    for (int e = 0; e < g.n_e; ++e) m += (wr[e] / R) * mac[r * g.n_e + e];
+    mu[row * g.n_lines + r] = m;
+  }
+}
+
+__device__ __forceinline__ void ray(const AbsGeom& g, int i, int j, int k, double pt[3], double d[3], double& lam,
+                                    double& dx) {
+#pragma clang fp contract(off)
+  pt[0] = g.pts[0][i];
+  pt[1] = g.pts[1][j];
+  pt[2] = g.pts[2][k];
+  for (int a = 0; a < 3; ++a) d[a] = g.det[a] - pt[a];
+  lam = (g.beam_x0 - pt[0]) / d[0];
+  if (!(lam > 0.0)) lam = 0.0;
+  dx = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * lam / 100.0;
+}
+
+__device__ __forceinline__ double sample_t(int s) { return s == 99 ? 1.0 : s * (1.0 / 99.0); }
+
+// One thread per eee point: ac[r][p] and the line weights W[r][p] = ac * noa.
+__global__ void absorb_kernel(AbsGeom g, const double* __restrict__ mu, const double* __restrict__ rho0,
+                              const double* __restrict__ atom_mass, const int* __restrict__ line_el,
+                              double* __restrict__ ac, double* __restrict__ W) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nq = (long)g.M[0] * g.M[1] * g.M[2];
+  if (q >= nq) return;
+  const int i = (int)(q % g.M[0]), j = (int)((q / g.M[0]) % g.M[1]), k = (int)(q / ((long)g.M[0] * g.M[1]));
+  double pt[3], d[3], lam, dx;
+  ray(g, i, j, k, pt, d, lam, dx);
+  const long emb = (long)g.n[0] * g.n[1] * g.n[2];
+  double acc[kMaxE];
+  for (int r = 0; r < g.n_lines; ++r) acc[r] = 0.0;
+  for (int s = 0; s < 100; ++s) {
+    const double tl = sample_t(s) * lam;
+    long row = 0;
+    bool inside = true;
+    long mult = 1;
+    for (int a = 0; a < 3; ++a) {
+      const double x = pt[a] + tl * d[a];
+      const double f = floor((x - g.edges[a][0]) * g.inv[a]);
+      int ia = 0;
+      if (f >= 0.0 && f < (double)g.n[a]) {
+        ia = (int)f;
+      } else {
+        inside = false;
+      }
+      row += ia * mult;
+      mult *= g.n[a];
+    }
+    if (!inside) row = emb;
+    for (int r = 0; r < g.n_lines; ++r) acc[r] -= mu[row * g.n_lines + r];
+  }
+  const long p = i + (long)g.X * (j + (long)g.Y * k);
+  for (int r = 0; r < g.n_lines; ++r) {
+    const double a = exp(acc[r] * dx);
+    const int e = line_el[r];
+    ac[r * g.V + p] = a;
+    W[r * g.V + p] = a * (rho0[e * g.V + p] / atom_mass[e]);
+  }
+}
+
+// noa_bar / atom mass at eee points ([e][V]) from the generation sums:
+// noa_bar[e] = sum over the lines of element e of ac * gen (summed over beams).
+__global__ void atoms_bar_kernel(AbsGeom g, const double* __restrict__ ac, const double* __restrict__ gen_b,
+                                 long gen_stride, int n_beams, const double* __restrict__ atom_mass,
+                                 const int* __restrict__ line_el, double* __restrict__ extra) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nq = (long)g.M[0] * g.M[1] * g.M[2];
+  if (q >= nq) return;
+  const int i = (int)(q % g.M[0]), j = (int)((q / g.M[0]) % g.M[1]), k = (int)(q / ((long)g.M[0] * g.M[1]));
+  const long p = i + (long)g.X * (j + (long)g.Y * k);
+  double nb[kMaxE];
+  for (int e = 0; e < g.n_e; ++e) nb[e] = 0.0;
+  for (int r = 0; r < g.n_lines; ++r) {
+    double gs = 0.0;
+    for (int b = 0; b < n_beams; ++b) gs += gen_b[b * gen_stride + r * g.V + p];
+    nb[line_el[r]] += ac[r * g.V + p] * gs;
+  }
+  for (int e = 0; e < g.n_e; ++e) extra[e * g.V + p] = nb[e] / atom_mass[e];
+}
+
+// compute_absorption_coefficient_pullback (x_ray_intensities.jl:131-153), with
+// its approximation: the march is traversed again with the factor fixed, and
+// at every sample inside the boxes the MassFraction pullback of
+// rho_bar = sum_r (-ac dx ac_bar)[r] mac[r] is added to that box (boxes found
+// by search on the edges; a sample on the last edge belongs to the last box).
+// ac_bar = noa[el(r)] * gen.  Samples from many points land in one box, so
+// the additions are atomic.
+__global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, const double* __restrict__ rho_pure,
+                                       const double* __restrict__ mac, const double* __restrict__ ac,
+                                       const double* __restrict__ rho0, const double* __restrict__ atom_mass,
+                                       const int* __restrict__ line_el, const double* __restrict__ gen_b,
+                                       long gen_stride, int n_beams, double* __restrict__ grad) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nq = (long)g.M[0] * g.M[1] * g.M[2];
+  if (q >= nq) return;
+  const int i = (int)(q % g.M[0]), j = (int)((q / g.M[0]) % g.M[1]), k = (int)(q / ((long)g.M[0] * g.M[1]));
+  const long p = i + (long)g.X * (j + (long)g.Y * k);
+  double pt[3], d[3], lam, dx;
+  ray(g, i, j, k, pt, d, lam, dx);
+  double rb[kMaxE];
+  for (int e = 0; e < g.n_e; ++e) rb[e] = 0.0;
+  for (int r = 0; r < g.n_lines; ++r) {
+    double gs = 0.0;
+    for (int b = 0; b < n_beams; ++b) gs += gen_b[b * gen_stride + r * g.V + p];
+    const int el = line_el[r];
+    const double ac_bar = (rho0[el * g.V + p] / atom_mass[el]) * gs;
+    const double a_ = -ac[r * g.V + p] * dx * ac_bar;
+    for (int e = 0; e < g.n_e; ++e) rb[e] += a_ * mac[r * g.n_e + e];
+  }
+  for (int s = 0; s < 100; ++s) {
+    const double tl = sample_t(s) * lam;
+    long cell = 0, mult = 1;
+    bool inside = true;
+    for (int a = 0; a < 3; ++a) {
+      const double x = pt[a] + tl * d[a];
+      const double* __restrict__ ed = g.edges[a];
+      const int n = g.n[a];
+      int ia;
+      if (x == ed[n]) {
+        ia = n - 1;
+      } else {
+        // searchsorted(edges, x, side="right") - 1
+        int lo = 0, hi = n + 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (ed[mid] <= x)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        ia = lo - 1;
+      }
+      if (ia < 0 || ia > n - 1) inside = false;
+      cell += (long)(ia < 0 ? 0 : ia) * mult;
+      mult *= n;
+    }
+    if (!inside) continue;
+    const double* wc = w + cell * g.n_e;
+    double R = 0.0, dot = 0.0;
+    for (int e = 0; e < g.n_e; ++e) R += wc[e] / rho_pure[e];
+    for (int e = 0; e < g.n_e; ++e) dot += wc[e] * rb[e];
+    const double R_bar = -dot / (R * R);
+    for (int e = 0; e < g.n_e; ++e) atomicAdd(&grad[cell * g.n_e + e], rb[e] / R + R_bar / rho_pure[e]);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], const long rho_off[8],
@@ -157,6 +364,42 @@ hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], co
   const long n_cells = (long)L.Mx * L.My * L.Mz;
   hipLaunchKernelGGL(pullback_kernel, dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0, s, g, w, rho_pure,
                      rho_bar, rb_stride, n_beams, extra_eee, grad);
+  return hipGetLastError();
+}
+
+
+hipError_t launch_absorption(const AbsorptionGeometry& a, const Layout& L, const double* w, const double* emb,
+                             const double* rho_pure, const double* mac, const double* rho, const double* atom_mass,
+                             const int* line_el, double* mu_scratch, double* ac, double* W, hipStream_t s) {
+  if (L.n_e > kMaxE || L.n_lines > kMaxE) return hipErrorInvalidValue;
+  AbsGeom g = abs_geom(a, L);
+  const long rows = (long)g.n[0] * g.n[1] * g.n[2] + 1;
+  hipLaunchKernelGGL(mu_table_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, g, w, emb, rho_pure, mac,
+                     mu_scratch);
+  const long nq = (long)L.Mx * L.My * L.Mz;
+  hipLaunchKernelGGL(absorb_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, mu_scratch, rho, atom_mass,
+                     line_el, ac, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_atoms_bar(const AbsorptionGeometry& a, const Layout& L, const double* ac, const double* gen_b,
+                            long gen_stride, int n_beams, const double* atom_mass, const int* line_el, double* extra,
+                            hipStream_t s) {
+  AbsGeom g = abs_geom(a, L);
+  const long nq = (long)L.Mx * L.My * L.Mz;
+  hipLaunchKernelGGL(atoms_bar_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, ac, gen_b, gen_stride,
+                     n_beams, atom_mass, line_el, extra);
+  return hipGetLastError();
+}
+
+hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* w,
+                                      const double* rho_pure, const double* mac, const double* ac, const double* rho,
+                                      const double* atom_mass, const int* line_el, const double* gen_b,
+                                      long gen_stride, int n_beams, double* grad, hipStream_t s) {
+  AbsGeom g = abs_geom(a, L);
+  const long nq = (long)L.Mx * L.My * L.Mz;
+  hipLaunchKernelGGL(absorb_pullback_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, w, rho_pure,
+                     mac, ac, rho, atom_mass, line_el, gen_b, gen_stride, n_beams, grad);
   return hipGetLastError();
 }
 

@@ -17,7 +17,9 @@ import numpy as np
 from . import _lib
 from ._lib import Desc, check, dptr, lib
 from .material import CellFractions, MassFractionMaterial
-from .problem import ForwardPNProblem, SolveInputs, absorption_factors, absorption_pullback, solve_inputs
+from . import physics
+from .problem import (ForwardPNProblem, SolveInputs, absorption_factors, absorption_pullback, el_index,
+                      solve_inputs)
 
 
 class Context:
@@ -103,6 +105,7 @@ class DeviceProblem:
     def set_material(self, inputs: SolveInputs):
         self.inputs = inputs
         self.cells = None
+        self.has_absorption = False
         r8 = (_lib._dp * 8)(*[dptr(np.ascontiguousarray(r)) for r in inputs.rho])
         check(lib().pnx_problem_set_material(self.handle, r8, dptr(np.ascontiguousarray(inputs.W))))
 
@@ -118,6 +121,36 @@ class DeviceProblem:
         rho_pure = np.ascontiguousarray(material.rho_pure, dtype=np.float64)
         check(lib().pnx_problem_set_cell_fractions(self.handle, dptr(w), dptr(rho_pure),
                                                    dptr(np.ascontiguousarray(inputs.W))))
+
+    def set_absorption(self, material: MassFractionMaterial, detector) -> np.ndarray:
+        """Absorption factors and line weights computed on the device
+        (after set_cell_fractions); returns the factors (P_eee, n_lines)."""
+        p = self.problem
+        g = p.gspec
+        mac = np.ascontiguousarray([[physics.mass_absorption(x, e) for e in material.elements] for x in p.x_rays],
+                                   dtype=np.float64)
+        am = np.ascontiguousarray([e.atom_mass_g for e in material.elements], dtype=np.float64)
+        el = np.ascontiguousarray(el_index(material.elements, p.x_rays), dtype=np.int32)
+        det = np.ascontiguousarray(detector, dtype=np.float64)
+        pts = [np.ascontiguousarray(a.points(0), dtype=np.float64) for a in g.axes]
+        edges = [np.ascontiguousarray(e, dtype=np.float64) for e in material.param.edges]
+        emb = np.ascontiguousarray(material.param.embedding, dtype=np.float64)
+        ac = np.zeros((int(np.prod(g.class_size(0))), len(p.x_rays)))
+        self._abs_keep = [mac, am, el, det, pts, edges, emb]
+        check(lib().pnx_problem_set_absorption(
+            self.handle, dptr(mac), dptr(am), el.ctypes.data_as(C.POINTER(C.c_int32)), dptr(det),
+            float(p.beam_positions[0][0]), (_lib._dp * 3)(*[dptr(x) for x in pts]),
+            (_lib._dp * 3)(*[dptr(x) for x in edges]), dptr(emb), dptr(ac)))
+        self.has_absorption = True
+        return ac
+
+    def material_gradient(self) -> np.ndarray:
+        """The whole material gradient on the device after forward_adjoint,
+        flattened like CellFractions.params."""
+        nx, ny, nz, ne = self.cells
+        g = np.zeros((nx * ny * nz, ne))
+        check(lib().pnx_problem_material_gradient(self.handle, dptr(g)))
+        return g.reshape(nz, ny, nx, ne).transpose(2, 1, 0, 3).ravel()
 
     def cell_fraction_gradient(self, extra_eee=None) -> np.ndarray:
         """d(rho_bar . rho)/d(cell fractions) on the device, flattened like
@@ -219,7 +252,11 @@ def material_gradient(problem: ForwardPNProblem, material, detector, si: SolveIn
 
     With `device` (a DeviceProblem whose material was set by set_cell_fractions
     and which has run forward_adjoint), the density and number-of-atoms terms
-    are pulled back on the device and rho_bar is not needed on the host."""
+    are pulled back on the device and rho_bar is not needed on the host; if
+    the absorption was also set on the device (set_absorption), the whole
+    gradient is computed there."""
+    if device is not None and getattr(device, "has_absorption", False):
+        return device.material_gradient()
     g = problem.gspec
     ac_bar = si.noa[:, si.el_idx] * gen
     noa_bar = np.zeros_like(si.noa)
@@ -249,6 +286,7 @@ def calc_intensities_pullback(problem: ForwardPNProblem, material, detector, ckp
     on_device = cell_aligned(material, problem.gspec)
     if on_device:
         dp.set_cell_fractions(material, si)
+        dp.set_absorption(material, detector)
     I = dp.forward()
 
     def pullback(I_bar):

@@ -90,3 +90,39 @@ def test_calc_intensities_pullback_device_path():
     e = rel_err(g_dev, g_host)
     print(f"material gradient device vs host rel err {e:.3e}", flush=True)
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("M", [(40, 1, 1), (20, 14, 1), (10, 8, 6)], ids=["1d", "2d", "3d"])
+def test_device_absorption_factors_match_host(M):
+    p, mat = _problem(M)
+    det = workloads.detector_40deg()
+    si = solve_inputs(p, mat, det)
+    dp = DeviceProblem(p, si, adjoint=False)
+    dp.set_cell_fractions(mat, si)
+    ac = dp.set_absorption(mat, det)
+    e = rel_err(ac, si.ac)
+    print(f"absorption factors device vs host rel err {e:.3e} (min factor {si.ac.min():.3f})", flush=True)
+    assert e < 1e-13
+    # intensities with device line weights agree with the host-weighted solve
+    I_dev = dp.forward()
+    I_host = DeviceProblem(p, si, adjoint=False).forward()
+    assert rel_err(I_dev, I_host) < 1e-13
+
+
+@pytest.mark.parametrize("M", [(40, 1, 1), (20, 14, 1), (10, 8, 6)], ids=["1d", "2d", "3d"])
+def test_device_material_gradient_matches_host(M):
+    p, mat = _problem(M, n_beams=2)
+    det = workloads.detector_40deg()
+    si = solve_inputs(p, mat, det)
+    ib = np.random.default_rng(9).uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
+    dp = DeviceProblem(p, si, ckpt_every=8)
+    dp.set_cell_fractions(mat, si)
+    dp.set_absorption(mat, det)
+    _, _, gen = dp.forward_adjoint(ib, want_rho_bar=False)
+    g_dev = dp.material_gradient()
+    hp = DeviceProblem(p, si, ckpt_every=8)
+    _, rb, gen_h = hp.forward_adjoint(ib)
+    g_host = solver.material_gradient(p, mat, det, si, rb, gen_h)
+    e = rel_err(g_dev, g_host)
+    print(f"whole material gradient device vs host rel err {e:.3e}", flush=True)
+    assert e < 1e-11
