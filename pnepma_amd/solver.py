@@ -109,18 +109,25 @@ class DeviceProblem:
         r8 = (_lib._dp * 8)(*[dptr(np.ascontiguousarray(r)) for r in inputs.rho])
         check(lib().pnx_problem_set_material(self.handle, r8, dptr(np.ascontiguousarray(inputs.W))))
 
-    def set_cell_fractions(self, material: MassFractionMaterial, inputs: SolveInputs):
+    def set_cell_fractions(self, material: MassFractionMaterial, inputs: SolveInputs | None):
         """Densities of every class computed on the device from the per-cell
-        fractions of a grid-aligned box material (see cell_aligned)."""
+        fractions of a grid-aligned box material (see cell_aligned).  The line
+        weights come from inputs.W, or, with inputs None, from a following
+        set_absorption."""
         if not cell_aligned(material, self.problem.gspec):
             raise ValueError("the material's boxes are not the grid's cells")
         self.inputs = inputs
+        self.has_absorption = False
         vals = material.param.vals                        # (nx, ny, nz, n_e)
         self.cells = vals.shape
         w = np.ascontiguousarray(vals.transpose(2, 1, 0, 3).reshape(-1, vals.shape[3]))  # x fastest
         rho_pure = np.ascontiguousarray(material.rho_pure, dtype=np.float64)
-        check(lib().pnx_problem_set_cell_fractions(self.handle, dptr(w), dptr(rho_pure),
-                                                   dptr(np.ascontiguousarray(inputs.W))))
+        if inputs is not None:
+            W = np.ascontiguousarray(inputs.W)
+        else:
+            g = self.problem.gspec
+            W = np.zeros((int(np.prod(g.class_size(0))), len(self.problem.x_rays)))
+        check(lib().pnx_problem_set_cell_fractions(self.handle, dptr(w), dptr(rho_pure), dptr(W)))
 
     def set_absorption(self, material: MassFractionMaterial, detector) -> np.ndarray:
         """Absorption factors and line weights computed on the device
