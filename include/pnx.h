@@ -222,6 +222,14 @@ int pnx_server_drain(pnx_server* s);
 int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active, int64_t* peak_bytes);
 /* Device bytes a request would hold while admitted. */
 int64_t pnx_request_device_bytes(const pnx_desc* desc);
+/* The operation sequence the server runs for a request with n energy steps and
+ * checkpoints every K, for inspection: for operation t < 3n + 1,
+ * out[9t .. 9t+8] = kind (1 forward, 2 recompute, 3 reverse, 4 final),
+ * energy step, checkpoint slot written before the step (-1: none), source
+ * buffer and slot, destination buffer and slot (0 state, 1 checkpoints,
+ * 2 segment), first (the step starts a forward run), fuse (the step ends with
+ * the fused odd pair).  Returns 3n + 1 (also when out is NULL). */
+int pnx_march_sequence(int32_t n, int32_t K, int32_t* out, int32_t capacity);
 
 /* Matrices the kernels use, for inspection: the coefficient buffer of order N
  * (adjoint = 0 forward, 1 adjoint): all advection nonzeros, then dense

@@ -966,6 +966,21 @@ int pnx_server_destroy(pnx_server* s) {
 
 double pnx_server_clock(const pnx_server* s) { return s ? s->now() : 0.0; }
 
+int pnx_march_sequence(int32_t n, int32_t K, int32_t* out, int32_t capacity) {
+  if (n < 1 || K < 1) return set_error(PNX_ERR_ARG, "n and K must be positive");
+  const int ops = march_ops(n);
+  if (!out) return ops;
+  if (capacity < 9 * ops) return set_error(PNX_ERR_ARG, "output too small: 9 * (3n + 1) integers needed");
+  const int Kc = std::min(K, n), nck = (n + Kc - 1) / Kc;
+  MarchPos pos = march_start();
+  for (int t = 0; t < ops; ++t) {
+    const MarchOp o = march_next(pos, n, Kc, nck);
+    const int32_t v[9] = {o.kind, o.step, o.ckpt_copy, o.src_buf, o.src_idx, o.dst_buf, o.dst_idx, o.first, o.fuse};
+    for (int i = 0; i < 9; ++i) out[9 * t + i] = v[i];
+  }
+  return ops;
+}
+
 int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   if (!s || !q || !ticket) return set_error(PNX_ERR_ARG, "null argument");
   if (q->desc.n_beams != 1) return set_error(PNX_ERR_ARG, "served requests carry exactly one beam");

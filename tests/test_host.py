@@ -133,3 +133,45 @@ def test_mass_fraction_pullback_fd():
         pm[q] -= h
         fd = ((m.with_params(pp).densities(pts) - m.with_params(pm).densities(pts)) * rbar).sum() / (2 * h)
         assert g[q] == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+def _expected_sequence(n, K):
+    """The operation order of the reference pullback with checkpoints
+    (pn_intensities.jl:366-404 with the tape replaced by checkpoints every K
+    steps), as the solver's loops run it, with the fused-pair flags of the
+    forward runs: a run is one checkpoint segment of the forward pass or one
+    recompute; its first step starts with the first odd half step, every step
+    but its last ends with the fused odd pair."""
+    K = min(K, n)
+    nck = -(-n // K)
+    ops = []
+    for k in range(1, n + 1):
+        seg_start = (k - 1) % K == 0
+        ops.append((1, k, (k - 1) // K if seg_start else -1, 0, 0, 0, 0, int(seg_start), int(k < n and k % K != 0)))
+    for s in range(nck - 1, -1, -1):
+        k0, k1 = s * K, min(n, s * K + K)
+        for k in range(k0 + 1, k1 + 1):
+            src = (1, s) if k == k0 + 1 else (2, k - k0 - 2)
+            ops.append((2, k, -1, *src, 2, k - k0 - 1, int(k == k0 + 1), int(k < k1)))
+        for i in range(k1, k0, -1):
+            ops.append((3, i, -1, 2, i - k0 - 1, 0, 0, 0, 0))
+    ops.append((4, 0, -1, 1, 0, 0, 0, 0, 0))
+    return ops
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 16, 33, 60])
+def test_served_operation_sequence(n):
+    L = _lib.lib()
+    for K in sorted({1, 2, 3, 5, max(1, int(np.ceil(np.sqrt(n)))), n, n + 3}):
+        m = L.pnx_march_sequence(n, K, None, 0)
+        assert m == 3 * n + 1
+        buf = (C.c_int32 * (9 * m))()
+        assert L.pnx_march_sequence(n, K, buf, 9 * m) == m
+        got = [tuple(buf[9 * t + i] for i in range(9)) for t in range(m)]
+        # destinations of reverse/final ops are unused: compare what the op reads and writes
+        exp = _expected_sequence(n, K)
+        for t, (a, b) in enumerate(zip(got, exp)):
+            if a[0] in (3, 4):
+                assert a[:5] == b[:5], (n, K, t, a, b)
+            else:
+                assert a == b, (n, K, t, a, b)
