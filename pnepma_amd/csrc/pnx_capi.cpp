@@ -515,7 +515,7 @@ int enqueue_forward(pnx_problem* p) {
       p, 1, p->d.n_steps, st, [&](int) { return st; },
       [&](int k) { return p->partial.p + (long)k * p->g.partial_step; });
   if (rc) return rc;
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles * kWavesPerBlock, p->rec_f.p, p->g.rec_len,
                                   p->g.sigma_off, p->d.n_steps, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams,
                                   p->I_dev.p, s));
   return PNX_OK;
@@ -608,7 +608,7 @@ int enqueue_forward_adjoint(pnx_problem* p) {
     HIP_TRY(launch_copy(p->ckpt.p + (long)sgi * st, state, st, s));
     if ((rc = forward_run(p, k0 + 1, k1, state, [&](int) { return state; }, part_k))) return rc;
   }
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p, p->g.rec_len,
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles * kWavesPerBlock, p->rec_f.p, p->g.rec_len,
                                   p->g.sigma_off, n, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams, p->I_dev.p, s));
   HIP_TRY(launch_fill(p->lam.p, st, 0.0, s));
   HIP_TRY(launch_fill(p->rho_bar_b.p, (long)p->rho_bar_b.n, 0.0, s));
@@ -686,7 +686,7 @@ int enqueue_forward_adjoint_exact(pnx_problem* p) {
                              hipMemcpyDeviceToDevice, s));
     if ((rc = forward_step(p, k, p->state.p, p->state.p, p->partial.p + (long)k * p->g.partial_step))) return rc;
   }
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles, p->rec_f.p,
+  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles * kWavesPerBlock, p->rec_f.p,
                                   p->g.rec_len, p->g.sigma_off, n, p->d.n_lines, dE, p->L.dvol, p->d.n_beams,
                                   p->I_dev.p, s));
   HIP_TRY(launch_fill(p->lam.p, st, 0.0, s));

@@ -96,7 +96,7 @@ int make_geometry(const pnx_desc& d, Geometry& g, std::string& err) {
   g.K = d.ckpt_every > 0 ? std::min(d.ckpt_every, d.n_steps) : d.n_steps;
   g.nck = (d.n_steps + g.K - 1) / g.K;
   g.prof_stride = 8L * L.Y * L.Z;
-  g.partial_step = (long)L.n_tiles * d.n_lines;
+  g.partial_step = (long)L.n_tiles * kWavesPerBlock * d.n_lines;
   g.partial_beam = (long)(d.n_steps + 1) * g.partial_step;
   return PNX_OK;
 }

@@ -195,12 +195,12 @@ __global__ void beam_sum_kernel(const double* __restrict__ in, long stride, int 
 
 // One block per beam (the body is shared with the march kernel).
 __global__ __launch_bounds__(kBlock) void reduce_intensity_kernel(const double* __restrict__ partial, long step_stride,
-                                                                  long beam_stride, int n_tiles,
+                                                                  long beam_stride, int n_parts,
                                                                   const double* __restrict__ rec, int rec_len,
                                                                   int sigma_off, int n_steps, int n_lines, double dE,
                                                                   double dvol, double* __restrict__ I_out) {
   const int b = blockIdx.x;
-  reduce_intensity_body(partial + b * beam_stride, step_stride, n_tiles, rec, rec_len, sigma_off, n_steps, n_lines, dE,
+  reduce_intensity_body(partial + b * beam_stride, step_stride, n_parts, rec, rec_len, sigma_off, n_steps, n_lines, dE,
                         dvol, I_out + b * n_lines);
 }
 
@@ -233,11 +233,11 @@ hipError_t launch_beam_sum(const double* in, long stride, int n_beams, long n, d
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,
+hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_parts,
                                    const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
                                    double dE, double dvol, int n_beams, double* I_out, hipStream_t s) {
   hipLaunchKernelGGL(reduce_intensity_kernel, dim3(n_beams), dim3(kBlock), 0, s, partial, step_stride, beam_stride,
-                     n_tiles, fwd_rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol, I_out);
+                     n_parts, fwd_rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol, I_out);
   return hipGetLastError();
 }
 

@@ -37,6 +37,10 @@ constexpr int kBlock = 256;
 // covers whole 128-byte lines (two row segments of 32 doubles).
 constexpr int tile_x(int dims) { return dims == 1 ? kBlock : 32; }
 
+// The forward even half step leaves one intensity partial sum per wavefront
+// (no block-wide reduction, no barrier): kBlock / 64 per tile, per line.
+constexpr int kWavesPerBlock = kBlock / 64;
+
 struct HalfStepArgs {
   Layout L;
   const double* from;     // state read for the other parity (beam 0)
@@ -51,7 +55,7 @@ struct HalfStepArgs {
   const double* mms;      // [c][16] beam moments
   const double* W;        // [r][V] line weights at eee points
   const double* ibar;     // [b][n_lines] adjoint seed (adjoint eee source), may be null
-  double* partial;        // [b][n_tiles][n_lines] intensity partial sums of this step (forward, even)
+  double* partial;        // [b][n_tiles * kWavesPerBlock][n_lines] intensity partial sums of this step (forward, even)
   long partial_stride;    // doubles per beam in partial
   double dt;
   int n_beams;
@@ -181,8 +185,9 @@ hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout&
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
 hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);
-// I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps
-hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_tiles,
+// I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps,
+// t over the n_parts = n_tiles * kWavesPerBlock partial sums of a step
+hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_parts,
                                    const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
                                    double dE, double dvol, int n_beams, double* I_out, hipStream_t s);
 // out[i] = sum_b in[b*stride + i], i < n

@@ -214,10 +214,12 @@ __device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restric
   });
 }
 
+// u0 receives the updated zeroth moment of eee (forward even half step), from
+// which the block forms the intensity contributions.
 template <int N, int DIM, int C, int MODE, int PAR>
 __device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
                                              gcd __restrict__ from, gcd src, gd dst, int i, int j, int k, long p,
-                                             int beam, double* contrib) {
+                                             int beam, double* u0) {
   // src and dst are the same buffer for in-place steps: the own values are
   // read before they are overwritten, so neither pointer is restrict.
   constexpr int NC = ncls<N, C>();
@@ -231,9 +233,7 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restri
   gd out = dst + L.cls_off[C];
 #pragma unroll
   for (int m = 0; m < NC; ++m) out[m * V + p] = un[m];
-  if constexpr (MODE == kForward && PAR == 0 && C == 0) {
-    for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) contrib[r] = glb(a.W)[r * V + p] * un[0];
-  }
+  if constexpr (MODE == kForward && PAR == 0 && C == 0) *u0 = un[0];
 }
 
 // Two consecutive odd-parity updates of class C at point p: the even classes
@@ -267,11 +267,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Intensities of one beam from the per-step, per-tile partial sums
+// Intensities of one beam from the per-step partial sums (n_parts per step)
 // (integrate_step_and_add!, x_ray_intensities.jl:172-191): thread t sums the
 // steps k = 1 + t, 1 + t + 256, ..., then the block reduces.  Every thread of
 // the block must call it.
-__device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int n_tiles,
+__device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int n_parts,
                                                       const double* __restrict__ rec, int rec_len, int sigma_off,
                                                       int n_steps, int n_lines, double dE, double dvol,
                                                       double* __restrict__ I_out) {
@@ -284,7 +284,7 @@ __device__ __forceinline__ void reduce_intensity_body(const double* __restrict__
     gcd __restrict__ sig = glb(rec) + (long)kk * rec_len + sigma_off;
     for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
       double integral = 0.0;
-      for (int t = 0; t < n_tiles; ++t) integral += ps[(long)t * n_lines + r];
+      for (int t = 0; t < n_parts; ++t) integral += ps[(long)t * n_lines + r];
       acc[r] += dE * dvol * sig[r] * integral * 1.0;
     }
   }
@@ -346,9 +346,9 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const Step
   if (!valid) i = j = k = 0;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
-  double contrib[kMaxLines];
-#pragma unroll
-  for (int r = 0; r < kMaxLines; ++r) contrib[r] = 0.0;
+  // eee's updated zeroth moment; the line contributions W u0 are formed in the
+  // reduction, so only this value stays live across the other class's update
+  double u0 = 0.0;
   if (valid) {
     static_for<0, 8>([&](auto Cc) {
       constexpr int C = decltype(Cc)::value;
@@ -357,25 +357,19 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const Step
                                 (DIM >= 3 && odd_in(C, 2) && k == 0));
         if (interior)
           update_class<N, DIM, C, MODE, PAR>(a, glb(v.rec), v.dt, glb(v.from) + bo, glb(v.src) + bo,
-                                             glb(v.dst) + bo, i, j, k, p, beam, contrib);
+                                             glb(v.dst) + bo, i, j, k, p, beam, &u0);
       }
     });
   }
   if constexpr (MODE == kForward && PAR == 0) {
-    __shared__ double red[kBlock / 64][kMaxLines];
+    // one partial sum per wavefront and line (no barrier)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    gcd __restrict__ Wp = glb(a.W);
+    gd __restrict__ out = glb(v.partial) + beam * a.partial_stride + ((long)tile * kWavesPerBlock + wave) * L.n_lines;
     for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) {
-      const double s = wave_sum(contrib[r]);
-      if (lane == 0) red[wave][r] = s;
+      const double s = wave_sum(valid ? Wp[r * L.V + p] * u0 : 0.0);
+      if (lane == 0) out[r] = s;
     }
-    __syncthreads();
-    if (threadIdx.x < L.n_lines && threadIdx.x < kMaxLines) {
-      double s = 0.0;
-#pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-      glb(v.partial)[beam * a.partial_stride + (long)tile * L.n_lines + threadIdx.x] = s;
-    }
-    __syncthreads();
   }
 }
 

@@ -702,7 +702,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       if (launch_slot(t, sl)) return -1;
     for (Request* r : sp.reduce) {
       const Geometry& g = r->g;
-      if (launch_reduce_intensity(r->partial, g.partial_step, g.partial_beam, g.L.n_tiles, r->rec_f, g.rec_len,
+      if (launch_reduce_intensity(r->partial, g.partial_step, g.partial_beam, g.L.n_tiles * kWavesPerBlock, r->rec_f, g.rec_len,
                                   g.sigma_off, r->d.n_steps, r->d.n_lines, g.dE, g.L.dvol, 1, r->I_dev, st) !=
           hipSuccess)
         return -1;
