@@ -93,7 +93,7 @@ def roofline_probe(reps=50):
                               "calibrated on a 1 GiB copy (profiles/half_step_p7_2d_128x96_160beams.md)"}
 
 
-def serve_run(preps, rate, budget, max_active):
+def serve_run(preps, rate, budget, max_active, tape_fraction=None):
     """Submit every prepared request with Poisson arrivals of the given rate; returns
     (wall seconds from first arrival to last completion, latencies in seconds, stats)."""
     import numpy as np
@@ -103,12 +103,15 @@ def serve_run(preps, rate, budget, max_active):
     arr = np.cumsum(gaps)
     arr -= arr[0]
     srv = serving.Server(budget, max_active=max_active)
+    if tape_fraction is not None:
+        srv.set_tape_limit(int(tape_fraction * budget))
     t0 = srv.clock() + 0.05
     tickets = [srv.submit(pr, arrival_s=t0 + a) for pr, a in zip(preps, arr)]
     progress(f"submitted {len(tickets)} requests in {srv.clock() - t0 + 0.05:.3f} s")
     srv.drain()
     tm = [srv.wait(t) for t in tickets]
     stats = srv.stats()
+    stats["taped"] = sum(x.taped for x in tm)
     srv.close()
     lat = np.array([x.done_s - x.arrival_s for x in tm])
     wall = max(x.done_s for x in tm) - t0
@@ -116,6 +119,18 @@ def serve_run(preps, rate, budget, max_active):
     if bad:
         raise RuntimeError(f"{bad} served requests failed")
     return wall, lat, stats
+
+
+def latency_split(specs, lat, passes):
+    """p50/p99 latency (ms) of the 1D and of the 2D requests of the stream."""
+    import numpy as np
+    is2d = np.tile([s.problem.gspec.axes[1].M > 1 for s in specs], passes)
+    out = {}
+    for name, sel in (("1d", ~is2d), ("2d", is2d)):
+        if sel.any():
+            out[name] = {"n": int(sel.sum()), "p50": float(np.percentile(lat[sel], 50) * 1e3),
+                         "p99": float(np.percentile(lat[sel], 99) * 1e3)}
+    return out
 
 
 def sequential_run(specs, preps, max_seconds=20.0, max_requests=64):
@@ -170,17 +185,17 @@ def main_serve(args):
     max_active = 4096
     # capacity probe on the head of the stream, all arriving at once
     head = preps[:min(len(preps), 256)]
-    w0, _, _ = serve_run(head, 0.0, budget, max_active)
+    w0, _, _ = serve_run(head, 0.0, budget, max_active, args.tape_fraction)
     capacity = len(head) / w0
     progress(f"capacity probe: {len(head)} requests in {w0:.2f} s")
     rate = 1.2 * capacity
     for _ in range(max(0, args.warmup - 1)):
-        serve_run(preps[:64], rate, budget, max_active)
+        serve_run(preps[:64], rate, budget, max_active, args.tape_fraction)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     walls, lats, stats = [], [], None
     for _ in range(args.steps):
-        w, lat, stats = serve_run(preps, rate, budget, max_active)
+        w, lat, stats = serve_run(preps, rate, budget, max_active, args.tape_fraction)
         walls.append(w)
         lats.append(lat)
         progress(f"served pass: {len(preps)} requests in {w:.2f} s")
@@ -205,8 +220,11 @@ def main_serve(args):
                                "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, CFL-0.5 step counts, "
                                "Poisson arrivals at 1.2x measured capacity",
                    "requests": len(preps), "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
+                   "tape_limit": "0.75 of the budget (server default)" if args.tape_fraction is None
+                   else f"{args.tape_fraction} of the budget",
                    "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "latency_ms_by_grid": latency_split(specs, lat, args.steps),
         "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "sample": f"first {len(seq_t)} requests of "
                            "the stream, one at a time, same kernels, no batching, no graph",
                            "mean_ms": float(seq_t.mean() * 1e3)},
@@ -289,6 +307,8 @@ def main():
     ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "kernel"])
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tape-fraction", type=float, default=None,
+                    help="share of the HBM budget the server may hold in taped requests (0: always checkpoint)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "

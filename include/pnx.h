@@ -207,12 +207,25 @@ typedef struct {
   double done_s;
   int32_t status;     /* PNX_OK or an error code */
   int32_t ticks;      /* scheduler ticks the request took part in */
+  int32_t taped;      /* 1: admitted taped (every forward state kept, no recompute) */
 } pnx_timing;
 
 /* Inputs are copied at submit; output pointers must stay valid until done. */
 int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_active, pnx_server** out);
 int pnx_server_destroy(pnx_server* s);
 double pnx_server_clock(const pnx_server* s);
+/* Memory-for-work trade of the admission: a request is admitted taped (its n
+ * forward states kept in HBM, so the reverse sweep needs no recompute: 2n + 1
+ * state sweeps instead of 3n + 1) while the arena's bytes in use plus its
+ * taped size stay within `bytes`.  Otherwise it is admitted checkpointed if
+ * its lane is short of work (fewer than 32 active 2D/3D or 128 active 1D
+ * requests), and waits for memory if not.  0 disables taping (every request
+ * checkpointed, admitted as soon as it fits).  Default: three quarters of the
+ * HBM budget.  Results agree with the
+ * checkpointed march to rounding (the fused odd pairs run over one run
+ * instead of one run per segment).  No counterpart in the reference, whose
+ * pullback always keeps the whole tape (pn_solver/pn_intensities.jl:353-404). */
+int pnx_server_set_tape_limit(pnx_server* s, int64_t bytes);
 int pnx_server_submit(pnx_server* s, const pnx_request* req, int64_t* ticket);
 int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t);
 int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t);
@@ -223,12 +236,13 @@ int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, in
 /* Device bytes a request would hold while admitted. */
 int64_t pnx_request_device_bytes(const pnx_desc* desc);
 /* The operation sequence the server runs for a request with n energy steps and
- * checkpoints every K, for inspection: for operation t < 3n + 1,
+ * checkpoints every K (K = 0: taped, 2n + 1 operations), for inspection: for
+ * operation t < 3n + 1,
  * out[9t .. 9t+8] = kind (1 forward, 2 recompute, 3 reverse, 4 final),
  * energy step, checkpoint slot written before the step (-1: none), source
  * buffer and slot, destination buffer and slot (0 state, 1 checkpoints,
  * 2 segment), first (the step starts a forward run), fuse (the step ends with
- * the fused odd pair).  Returns 3n + 1 (also when out is NULL). */
+ * the fused odd pair).  Returns the operation count (also when out is NULL). */
 int pnx_march_sequence(int32_t n, int32_t K, int32_t* out, int32_t capacity);
 
 /* Matrices the kernels use, for inspection: the coefficient buffer of order N

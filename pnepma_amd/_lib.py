@@ -29,7 +29,7 @@ class Request(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
-                ("ticks", C.c_int32)]
+                ("ticks", C.c_int32), ("taped", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol include/pnx.h declares
@@ -61,6 +61,7 @@ SIGNATURES = {
     "pnx_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "pnx_server_destroy": (C.c_int, [C.c_void_p]),
     "pnx_server_clock": (C.c_double, [C.c_void_p]),
+    "pnx_server_set_tape_limit": (C.c_int, [C.c_void_p, C.c_int64]),
     "pnx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "pnx_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(Timing)]),
     "pnx_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Timing)]),

@@ -20,6 +20,12 @@
 // A reverse step i runs as the density pullback of step i and the first odd
 // adjoint half step of step i in one fused pass, then the even and the last
 // odd adjoint half steps.
+//
+// nck == 0 selects the taped sequence (the reference's own tape): every
+// forward state is kept in K = n segment slots.  The forward runs once, out of
+// place from the zero initial state in kBufState into slots 0..n-1, as one
+// fused run; the reverse steps read their states there with no recompute, and
+// the final pullback reads u_0 from kBufState.  Total: 2n + 1 operations.
 #pragma once
 
 #include "pnx_kernels.hpp"
@@ -48,9 +54,10 @@ struct MarchOp {
 
 constexpr MarchPos march_start() { return MarchPos{kStFwd, 1, 0, 0}; }
 
-constexpr int march_ops(int n) { return 3 * n + 1; }
+constexpr int march_ops(int n, int nck) { return nck == 0 ? 2 * n + 1 : 3 * n + 1; }
 
-// Returns the operation at position s and advances s (n steps, checkpoints every K, nck of them).
+// Returns the operation at position s and advances s (n steps, checkpoints
+// every K, nck of them; nck == 0: taped, K == n).
 constexpr MarchOp march_next(MarchPos& s, int n, int K, int nck) {
   MarchOp o{kOpNone, 0, -1, kBufState, 0, kBufState, 0, kBufState, 0, 0, 0, 0};
   switch (s.stage) {
@@ -58,14 +65,33 @@ constexpr MarchOp march_next(MarchPos& s, int n, int K, int nck) {
       const int k = s.k;
       o.kind = kOpFwd;
       o.step = k;
-      o.ckpt_copy = (k - 1) % K == 0 ? (k - 1) / K : -1;
-      o.first = (k - 1) % K == 0;
-      o.fuse = k < n && k % K != 0;
+      if (nck == 0) {
+        if (k > 1) {
+          o.src_buf = kBufSeg;
+          o.src_idx = k - 2;
+        }
+        o.dst_buf = kBufSeg;
+        o.dst_idx = k - 1;
+        o.dst2_buf = kBufSeg;
+        o.dst2_idx = k;
+        o.first = k == 1;
+        o.fuse = k < n;
+      } else {
+        o.ckpt_copy = (k - 1) % K == 0 ? (k - 1) / K : -1;
+        o.first = (k - 1) % K == 0;
+        o.fuse = k < n && k % K != 0;
+      }
       if (k == n) {
         o.reduce = 1;
-        s.stage = kStRecompute;
-        s.segi = nck - 1;
-        s.k = s.segi * K + 1;
+        if (nck == 0) {
+          s.stage = kStReverse;
+          s.segi = 0;
+          s.i = n;
+        } else {
+          s.stage = kStRecompute;
+          s.segi = nck - 1;
+          s.k = s.segi * K + 1;
+        }
       } else {
         s.k = k + 1;
       }
@@ -118,7 +144,7 @@ constexpr MarchOp march_next(MarchPos& s, int n, int K, int nck) {
     case kStFinal:
       o.kind = kOpFinal;
       o.step = 0;
-      o.src_buf = kBufCkpt;
+      o.src_buf = nck == 0 ? kBufState : kBufCkpt;
       o.src_idx = 0;
       s.stage = kStDone;
       break;
@@ -138,7 +164,7 @@ struct MarchArgs {
   double* state;      // forward state
   double* lam;        // adjoint state
   double* ckpt;       // nck checkpoint states
-  double* seg;        // K recomputed states
+  double* seg;        // K recomputed states (taped: the n forward states)
   double* partial;    // [(n+1)][n_tiles][n_lines] intensity partials (slot 0: scratch)
   double* I_out;      // n_lines intensities
   const double* rec_f;

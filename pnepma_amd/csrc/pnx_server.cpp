@@ -57,6 +57,10 @@ constexpr int kSlotPair = 7;
 constexpr int kSlotRev = 8;
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
+// Active requests that keep a lane's launches full: beyond these a request
+// that cannot be taped waits for memory rather than run checkpointed.
+constexpr int kBusyGridLane = 32;
+constexpr int kBusyMarchLane = 128;
 // 1D, and up to four 2D/3D lanes.  (Six lanes, with orders 7 and 9 split by
 // grid size, measured slower than four: the grouped launches get smaller.)
 constexpr int kMaxLanes = 5;
@@ -198,8 +202,7 @@ class PinnedPool {
 
 // doubles each device buffer of a request needs
 struct Sizes {
-  long state, ckpt, seg, rho, W, rec, prof, partial, rb, gen, small;
-  long total() const { return 2 * state + ckpt + seg + rho + W + 3 * rec + prof + partial + rb + gen + small; }
+  long state, ckpt, seg, tape, rho, W, rec, prof, partial, rb, gen, small;
 };
 
 Sizes request_sizes(const Geometry& g, const pnx_desc& d) {
@@ -207,6 +210,7 @@ Sizes request_sizes(const Geometry& g, const pnx_desc& d) {
   s.state = g.L.beam_stride;
   s.ckpt = (long)g.nck * s.state;
   s.seg = (long)g.K * s.state;
+  s.tape = 0;
   s.rho = g.rho_total;
   s.W = (long)d.n_lines * g.L.V;
   s.rec = (long)(d.n_steps + 1) * g.rec_len;
@@ -218,14 +222,25 @@ Sizes request_sizes(const Geometry& g, const pnx_desc& d) {
   return s;
 }
 
+// The same request taped: no checkpoints or recompute slots, the n forward states instead.
+Sizes taped_sizes(const Sizes& z, int n) {
+  Sizes s = z;
+  s.ckpt = 0;
+  s.seg = 0;
+  s.tape = (long)n * z.state;
+  return s;
+}
+
 // Offsets (doubles, each buffer 256-byte aligned) of a request's buffers in
 // its arena block.  The inputs come first, in the same layout as the pinned
 // host block they are staged in, so admission is one copy; the buffers that
-// start at zero follow, so admission clears them with one fill.
+// start at zero follow, so admission clears them with one fill.  The tape is
+// not cleared: its slots are written before they are read, and the kernels
+// read its ghost slots only to replace them by rebuilt boundary values.
 struct Offsets {
   long rec_f, rec_a, rec_d, rho, W, prof, mms, ibar, in_end;
   long state, lam, rb, gen, seg, zero_end;
-  long ckpt, partial, I, total;
+  long ckpt, tape, partial, I, total;
 };
 
 Offsets request_offsets(const Sizes& z) {
@@ -252,6 +267,7 @@ Offsets request_offsets(const Sizes& z) {
   o.seg = take(z.seg);
   o.zero_end = p;
   o.ckpt = take(z.ckpt);
+  o.tape = take(z.tape);
   o.partial = take(z.partial);
   o.I = take(8);
   o.total = p;
@@ -267,6 +283,8 @@ struct Request {
   Sizes sz{};
   int lane = 0;
   Offsets off_{};
+  Offsets off_tape{};            // layout if admitted taped
+  size_t bytes_tape = 0;
   // pinned host block: the inputs packed in the device layout (off_.in_end
   // doubles), then the results (n_lines + rho_bar + gen doubles)
   double* pinned = nullptr;
@@ -314,6 +332,7 @@ struct pnx_server {
   pnx_context* ctx = nullptr;
   Arena arena;
   int64_t budget = 0;
+  int64_t tape_limit = 0;  // admit a request taped while the arena stays within this
   int max_active = 0;
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
@@ -383,7 +402,7 @@ int admit(pnx_server* s, Request& r) {
   r.lam = base + o.lam;
   r.rb = base + o.rb;
   r.gen = base + o.gen;
-  r.seg = base + o.seg;
+  r.seg = base + (r.g.nck == 0 ? o.tape : o.seg);
   r.ckpt = base + o.ckpt;
   r.partial = base + o.partial;
   r.I_dev = base + o.I;
@@ -393,7 +412,7 @@ int admit(pnx_server* s, Request& r) {
   // before they are read, but their ghost slots are kept finite
   if (launch_fill(r.state, o.zero_end - o.state, 0.0, st) != hipSuccess) return -1;
   r.pos = march_start();
-  r.ops_left = march_ops(r.d.n_steps);
+  r.ops_left = march_ops(r.d.n_steps, r.g.nck);
   return 0;
 }
 
@@ -805,11 +824,28 @@ void worker_loop(pnx_server* s) {
       if (s->error) break;
       const double now = s->now();
       // admission in arrival order while memory and slots allow
+      int lane_n[kMaxLanes];
+      for (int li = 0; li < kMaxLanes; ++li) lane_n[li] = (int)s->lanes[li].active.size();
       while (!s->pending.empty() && s->n_active < s->max_active) {
         auto& r = s->pending.front();
         if (r->timing.arrival_s > now) break;
-        const size_t off = s->arena.alloc(r->bytes);
+        // Taped (no recompute) while the tape fits under the tape limit.  Else
+        // checkpointed, but only into a lane short of work: a lane that is
+        // busy lets the request wait until memory frees for its tape.
+        size_t off = SIZE_MAX;
+        if (s->arena.used() + r->bytes_tape <= (size_t)s->tape_limit) off = s->arena.alloc(r->bytes_tape);
+        if (off != SIZE_MAX) {
+          r->off_ = r->off_tape;
+          r->bytes = r->bytes_tape;
+          r->g.K = r->d.n_steps;
+          r->g.nck = 0;
+          r->timing.taped = 1;
+        } else {
+          if (s->tape_limit > 0 && lane_n[r->lane] >= (r->lane == 0 ? kBusyMarchLane : kBusyGridLane)) break;
+          off = s->arena.alloc(r->bytes);
+        }
         if (off == SIZE_MAX) break;
+        lane_n[r->lane] += 1;
         r->off = off;
         r->timing.admit_s = now;
         admitted.push_back(r);
@@ -915,6 +951,7 @@ int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_ac
   auto* s = new pnx_server;
   s->ctx = ctx;
   s->budget = hbm_budget_bytes;
+  s->tape_limit = hbm_budget_bytes / 4 * 3;
   s->max_active = max_active;
   if (s->arena.init((size_t)hbm_budget_bytes)) {
     release_server(s);
@@ -966,12 +1003,19 @@ int pnx_server_destroy(pnx_server* s) {
 
 double pnx_server_clock(const pnx_server* s) { return s ? s->now() : 0.0; }
 
+int pnx_server_set_tape_limit(pnx_server* s, int64_t bytes) {
+  if (!s || bytes < 0) return set_error(PNX_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tape_limit = std::min(bytes, s->budget);
+  return PNX_OK;
+}
+
 int pnx_march_sequence(int32_t n, int32_t K, int32_t* out, int32_t capacity) {
-  if (n < 1 || K < 1) return set_error(PNX_ERR_ARG, "n and K must be positive");
-  const int ops = march_ops(n);
+  if (n < 1 || K < 0) return set_error(PNX_ERR_ARG, "n must be positive and K non-negative");
+  const int Kc = K == 0 ? n : std::min(K, n), nck = K == 0 ? 0 : (n + Kc - 1) / Kc;
+  const int ops = march_ops(n, nck);
   if (!out) return ops;
-  if (capacity < 9 * ops) return set_error(PNX_ERR_ARG, "output too small: 9 * (3n + 1) integers needed");
-  const int Kc = std::min(K, n), nck = (n + Kc - 1) / Kc;
+  if (capacity < 9 * ops) return set_error(PNX_ERR_ARG, "output too small: 9 integers per operation needed");
   MarchPos pos = march_start();
   for (int t = 0; t < ops; ++t) {
     const MarchOp o = march_next(pos, n, Kc, nck);
@@ -995,8 +1039,10 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   r->sz = request_sizes(r->g, r->d);
   r->lane = lane_of(r->d.N, r->g.L.dims, s->n_lanes - 1);
   r->off_ = request_offsets(r->sz);
+  r->off_tape = request_offsets(taped_sizes(r->sz, r->d.n_steps));
   const Offsets& o = r->off_;
   r->bytes = (size_t)o.total * sizeof(double);
+  r->bytes_tape = (size_t)r->off_tape.total * sizeof(double);
   if ((int64_t)r->bytes > s->budget) return set_error(PNX_ERR_MEMORY, "request exceeds the server's HBM budget");
   const Geometry& g = r->g;
   for (int c = 0; c < 8; ++c) {

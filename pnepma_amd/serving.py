@@ -147,6 +147,11 @@ class Server:
     def clock(self) -> float:
         return lib().pnx_server_clock(self.handle)
 
+    def set_tape_limit(self, nbytes: int):
+        """Admit requests taped (every forward state kept, no recompute) while the
+        arena's bytes in use stay within nbytes; 0 disables taping."""
+        check(lib().pnx_server_set_tape_limit(self.handle, int(nbytes)))
+
     def submit(self, prep: _Prepared, arrival_s: float = 0.0) -> int:
         p = prep.spec.problem
         r = Request()

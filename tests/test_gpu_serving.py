@@ -29,21 +29,33 @@ def _alone(spec):
     return I[0], rb, gen
 
 
-def test_served_equals_alone():
+@pytest.mark.parametrize("tape", ["default", "none", "all"])
+def test_served_equals_alone(tape):
+    """Checkpointed (recompute per segment) and taped (every forward state kept)
+    requests, alone and mixed under a budget that forces queueing."""
     specs = _specs()
     preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
     need = max(serving.request_device_bytes(s.problem) for s in specs)
-    srv = serving.Server(hbm_budget_bytes=3 * need, max_active=64)
+    budget = 3 * need if tape != "all" else 1 << 30
+    srv = serving.Server(hbm_budget_bytes=budget, max_active=64)
+    if tape == "none":
+        srv.set_tape_limit(0)
     tickets = []
     for n, pr in enumerate(preps):
         tickets.append(srv.submit(pr, arrival_s=0.002 * n))
+    taped = []
     for t in tickets:
         tm = srv.wait(t)
         assert tm.status == 0
+        taped.append(tm.taped)
     st = srv.stats()
-    print("server stats", st, flush=True)
+    print("server stats", st, "taped", taped, flush=True)
+    if tape == "none":
+        assert not any(taped)
+    if tape == "all":
+        assert all(taped)
     assert st["completed"] == len(specs)
-    assert st["peak_bytes"] <= 3 * need + 64 * 1024
+    assert st["peak_bytes"] <= budget
     srv.close()
     for spec, pr in zip(specs, preps):
         I, rb, gen = _alone(spec)
@@ -55,12 +67,15 @@ def test_served_equals_alone():
         assert rel_err(pr.gen, gen) < 1e-12
 
 
-def test_served_matches_oracle(oracle_lib):
+@pytest.mark.parametrize("taped", [True, False])
+def test_served_matches_oracle(oracle_lib, taped):
     specs = _specs(n=4, seed=5)
     preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
     srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=16)
+    if not taped:
+        srv.set_tape_limit(0)
     for t in [srv.submit(pr) for pr in preps]:
-        srv.wait(t)
+        assert srv.wait(t).taped == int(taped)
     srv.close()
     for spec, pr in zip(specs, preps):
         I, rb, acb, noab = oracle_lib.forward_adjoint(spec.problem, pr.si, spec.I_bar)

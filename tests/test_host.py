@@ -159,17 +159,33 @@ def _expected_sequence(n, K):
     return ops
 
 
+def _expected_taped_sequence(n):
+    """The reference pullback's own order (pn_intensities.jl:366-404), every
+    forward state kept: one forward run from the zero state (buffer 0) into
+    slots 0..n-1 of buffer 2, the reverse steps on those slots, the final
+    pullback on the zero state."""
+    ops = []
+    for k in range(1, n + 1):
+        src = (0, 0) if k == 1 else (2, k - 2)
+        ops.append((1, k, -1, *src, 2, k - 1, int(k == 1), int(k < n)))
+    for i in range(n, 0, -1):
+        ops.append((3, i, -1, 2, i - 1, 0, 0, 0, 0))
+    ops.append((4, 0, -1, 0, 0, 0, 0, 0, 0))
+    return ops
+
+
 @pytest.mark.parametrize("n", [1, 2, 7, 16, 33, 60])
 def test_served_operation_sequence(n):
     L = _lib.lib()
-    for K in sorted({1, 2, 3, 5, max(1, int(np.ceil(np.sqrt(n)))), n, n + 3}):
+    for K in sorted({0, 1, 2, 3, 5, max(1, int(np.ceil(np.sqrt(n)))), n, n + 3}):
         m = L.pnx_march_sequence(n, K, None, 0)
-        assert m == 3 * n + 1
+        assert m == (2 * n + 1 if K == 0 else 3 * n + 1)
         buf = (C.c_int32 * (9 * m))()
         assert L.pnx_march_sequence(n, K, buf, 9 * m) == m
         got = [tuple(buf[9 * t + i] for i in range(9)) for t in range(m)]
         # destinations of reverse/final ops are unused: compare what the op reads and writes
-        exp = _expected_sequence(n, K)
+        exp = _expected_taped_sequence(n) if K == 0 else _expected_sequence(n, K)
+        assert len(got) == len(exp)
         for t, (a, b) in enumerate(zip(got, exp)):
             if a[0] in (3, 4):
                 assert a[:5] == b[:5], (n, K, t, a, b)
