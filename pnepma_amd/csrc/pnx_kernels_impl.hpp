@@ -129,6 +129,15 @@ __device__ __forceinline__ void stencil(const Layout& L, gcd __restrict__ from, 
   });
 }
 
+// Per-point loops over elements and x-ray lines run in batches of kBatch: the
+// loads of a batch are issued together and waited for once (a loop whose
+// bound is a runtime value would wait for each load in turn).  Lanes of a
+// batch beyond the count load entry e0 again and are not used, so the sums
+// keep their order and their values.
+constexpr int kBatch = 4;
+
+__device__ __forceinline__ int batch_index(int e0, int b, int n) { return e0 + b < n ? e0 + b : e0; }
+
 // S and the per-degree C of a point (weighted_sum!, pn_problem.jl:213-220).
 template <int N>
 __device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne, gcd __restrict__ rec, double& S,
@@ -136,12 +145,20 @@ __device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne,
   S = 0.0;
 #pragma unroll
   for (int l = 0; l <= N; ++l) Cl[l] = 0.0;
-  for (int e = 0; e < ne; ++e) {
-    const double r = rc[e * V + p];
-    S += r * rec[e];
-    gcd __restrict__ ce = rec + ne + e * (N + 1);
+  for (int e0 = 0; e0 < ne; e0 += kBatch) {
+    double r[kBatch];
 #pragma unroll
-    for (int l = 0; l <= N; ++l) Cl[l] += r * ce[l];
+    for (int b = 0; b < kBatch; ++b) r[b] = rc[batch_index(e0, b, ne) * V + p];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const int e = e0 + b;
+      if (e < ne) {
+        S += r[b] * rec[e];
+        gcd __restrict__ ce = rec + ne + e * (N + 1);
+#pragma unroll
+        for (int l = 0; l <= N; ++l) Cl[l] += r[b] * ce[l];
+      }
+    }
   }
 }
 
@@ -195,7 +212,15 @@ __device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restric
       gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
       gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
       gcd __restrict__ Wp = glb(a.W);
-      for (int r = 0; r < L.n_lines; ++r) q0 -= sig[r] * Wp[r * V + p] * ib[r];
+      const int nl = L.n_lines;
+      for (int r0 = 0; r0 < nl; r0 += kBatch) {
+        double wv[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) wv[b] = Wp[batch_index(r0, b, nl) * V + p];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b)
+          if (r0 + b < nl) q0 -= sig[r0 + b] * wv[b] * ib[r0 + b];
+      }
     }
   }
   static_for<0, NC>([&](auto Ic) {
@@ -366,9 +391,18 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const Step
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     gcd __restrict__ Wp = glb(a.W);
     gd __restrict__ out = glb(v.partial) + beam * a.partial_stride + ((long)tile * kWavesPerBlock + wave) * L.n_lines;
-    for (int r = 0; r < L.n_lines && r < kMaxLines; ++r) {
-      const double s = wave_sum(valid ? Wp[r * L.V + p] * u0 : 0.0);
-      if (lane == 0) out[r] = s;
+    const int nl = L.n_lines < kMaxLines ? L.n_lines : kMaxLines;
+    for (int r0 = 0; r0 < nl; r0 += kBatch) {
+      double wv[kBatch];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) wv[b] = Wp[batch_index(r0, b, nl) * L.V + p];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        if (r0 + b < nl) {
+          const double s = wave_sum(valid ? wv[b] * u0 : 0.0);
+          if (lane == 0) out[r0 + b] = s;
+        }
+      }
     }
   }
 }
@@ -469,23 +503,41 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
   gcd __restrict__ s = rec;
   gcd __restrict__ cl = rec + ne;
   const double w = a.L.dvol;
-  for (int e = 0; e < ne; ++e) {
-    double acc = rb[e * V + p];
-    const double se = s[e];
-    static_for<0, NC>([&](auto Ic) {
-      constexpr int I = decltype(Ic)::value;
-      constexpr int l = Tab<N>::t.l[C][I];
-      acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
-    });
-    rb[e * V + p] = acc;
+  for (int e0 = 0; e0 < ne; e0 += kBatch) {
+    double acc[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) acc[b] = rb[batch_index(e0, b, ne) * V + p];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const int e = e0 + b;
+      if (e < ne) {
+        const double se = s[e];
+        static_for<0, NC>([&](auto Ic) {
+          constexpr int I = decltype(Ic)::value;
+          constexpr int l = Tab<N>::t.l[C][I];
+          acc[b] += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
+        });
+        rb[e * V + p] = acc[b];
+      }
+    }
   }
   if constexpr (C == 0) {
     gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
     gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
     gd __restrict__ g = glb(a.gen) + beam * a.gen_stride;
-    for (int r = 0; r < L.n_lines; ++r) {
-      const double integral = a.dE * w * sig[r] * int_fac * ib[r];
-      g[r * V + p] += u[0] * integral;
+    const int nl = L.n_lines;
+    for (int r0 = 0; r0 < nl; r0 += kBatch) {
+      double gv[kBatch];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) gv[b] = g[batch_index(r0, b, nl) * V + p];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        const int r = r0 + b;
+        if (r < nl) {
+          const double integral = a.dE * w * sig[r] * int_fac * ib[r];
+          g[r * V + p] = gv[b] + u[0] * integral;
+        }
+      }
     }
   }
 }
