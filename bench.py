@@ -9,6 +9,9 @@ budget.  A "step" is one pass of the whole stream; the timed region covers K
 passes with all request inputs prepared in host memory beforehand.  Reported
 beside it: p50/p99 latency (arrival to result), the plain one-request-at-a-time
 GPU loop on a sample of the same requests, and the C oracle on one core.
+The timed passes offer 1.5x the capacity measured on the head of the stream,
+so the server stays saturated and the throughput is its own; p50/p99 come
+from an untimed pass at 0.8x the measured throughput.
 
 raster2d_grad: 2D P7 128x96, 32 beams, forward+adjoint of the batch per step.
 
@@ -181,14 +184,17 @@ def main_serve(args):
     torch.zeros(1, device="cuda:0")
     from pnepma_amd import serving
     free, total = torch.cuda.mem_get_info()
-    budget = int(min(free * 0.7, 200e9))
+    budget = int(min(free * 0.8, 240e9))
     max_active = 4096
-    # capacity probe on the head of the stream, all arriving at once
-    head = preps[:min(len(preps), 256)]
+    # capacity probe on the head of the stream, all arriving at once; its tail
+    # is poorly batched, so it underestimates the sustained capacity
+    head = preps[:min(len(preps), 512)]
     w0, _, _ = serve_run(head, 0.0, budget, max_active, args.tape_fraction)
     capacity = len(head) / w0
     progress(f"capacity probe: {len(head)} requests in {w0:.2f} s")
-    rate = 1.2 * capacity
+    # the timed passes offer more than the server sustains, so it stays saturated
+    # and `value` is its throughput, not the arrival rate
+    rate = 1.5 * capacity
     for _ in range(max(0, args.warmup - 1)):
         serve_run(preps[:64], rate, budget, max_active, args.tape_fraction)
     torch.cuda.synchronize()
@@ -201,8 +207,14 @@ def main_serve(args):
         progress(f"served pass: {len(preps)} requests in {w:.2f} s")
     torch.cuda.synchronize()
     total_wall = time.perf_counter() - t0
-    lat = np.concatenate(lats)
+    lat_over = np.concatenate(lats)
     served = len(preps) * args.steps / sum(walls)
+    # latency at a load the server sustains: the same stream at 80 % of the
+    # measured throughput (untimed)
+    rate_lat = 0.8 * served
+    w_lat, lat, stats_lat = serve_run(preps, rate_lat, budget, max_active, args.tape_fraction)
+    progress(f"latency pass at {rate_lat:.1f}/s: {len(preps)} requests in {w_lat:.2f} s, "
+             f"p50 {np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms")
     # results of the served pass vs the same requests alone (plain loop)
     seq_t, seq_I = sequential_run(specs, preps)
     seq_rate = len(seq_t) / seq_t.sum()
@@ -218,13 +230,17 @@ def main_serve(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"serving stream of {len(preps)} forward+gradient requests: PN 3/5/7/9, "
                                "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, CFL-0.5 step counts, "
-                               "Poisson arrivals at 1.2x measured capacity",
+                               "Poisson arrivals at 1.5x the all-at-once capacity of its first 512 requests",
                    "requests": len(preps), "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
                    "tape_limit": "0.75 of the budget (server default)" if args.tape_fraction is None
                    else f"{args.tape_fraction} of the budget",
                    "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "latency_ms_by_grid": latency_split(specs, lat, args.steps),
+        "latency_load": {"arrival_rate_per_s": rate_lat, "note": "p50/p99 from an untimed pass of the same stream "
+                         "at 0.8x the measured throughput", "taped": stats_lat["taped"]},
+        "latency_ms_by_grid": latency_split(specs, lat, 1),
+        "saturated_pass_latency_ms": {"p50": float(np.percentile(lat_over, 50) * 1e3),
+                                      "p99": float(np.percentile(lat_over, 99) * 1e3)},
         "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "sample": f"first {len(seq_t)} requests of "
                            "the stream, one at a time, same kernels, no batching, no graph",
                            "mean_ms": float(seq_t.mean() * 1e3)},

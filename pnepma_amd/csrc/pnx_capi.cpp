@@ -592,6 +592,20 @@ DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
   return a;
 }
 
+const double* rec_adj(pnx_problem* p, int i) { return p->rec_a.p + (long)i * p->g.rec_len; }
+
+// Fused reverse pass at step i on u: the last odd adjoint half step of step
+// i + 1 first (if with_a3), the pullback with weight int_fac, the first odd
+// adjoint half step of step i last (if with_a1).
+RevArgs rev_args(pnx_problem* p, int i, const double* u, double int_fac, bool with_a3, bool with_a1) {
+  RevArgs ra{};
+  ra.adj = half_args(p, kAdjoint, rec_adj(p, i), p->g.dE / 2, p->lam.p, p->lam.p, p->lam.p, nullptr);
+  ra.dens = dens_args(p, i, u, int_fac);
+  ra.rec3 = with_a3 ? rec_adj(p, i + 1) : nullptr;
+  ra.a1 = with_a1 ? 1 : 0;
+  return ra;
+}
+
 // Forward pass with checkpoints, then the reverse sweep of the reference
 // pullback (pn_intensities.jl:381-404).
 int enqueue_forward_adjoint(pnx_problem* p) {
@@ -621,23 +635,17 @@ int enqueue_forward_adjoint(pnx_problem* p) {
              p, k0 + 1, k1, p->ckpt.p + (long)sgi * st, [&](int k) { return p->seg.p + (long)(k - k0 - 1) * st; },
              [&](int) { return p->partial.p; })))
       return rc;
-    // per reverse step i: dens(i) + A1(i) fused, then AE(i), A3(i)
-    const double h = p->g.dE / 2;
-    auto rec_a = [&](int i) { return p->rec_a.p + (long)i * p->g.rec_len; };
+    // per reverse step i: A3(i+1) + dens(i) + A1(i) fused, then AE(i)
     for (int i = k1; i > k0; --i) {
       const double* u = p->seg.p + (long)(i - k0 - 1) * st;
-      RevArgs ra{};
-      ra.adj = half_args(p, kAdjoint, rec_a(i), h, p->lam.p, p->lam.p, p->lam.p, nullptr);
-      ra.dens = dens_args(p, i, u, 1.0);
-      HIP_TRY(launch_rev(N, dims, ra, s));
+      HIP_TRY(launch_rev(N, dims, rev_args(p, i, u, 1.0, i < n, true), s));
       HIP_TRY(launch_half_step(N, dims, 0, kAdjoint,
-                               half_args(p, kAdjoint, rec_a(i), p->g.dE, p->lam.p, p->lam.p, p->lam.p, nullptr), s));
-      HIP_TRY(launch_half_step(N, dims, 1, kAdjoint,
-                               half_args(p, kAdjoint, rec_a(i), h, p->lam.p, p->lam.p, p->lam.p, nullptr), s));
+                               half_args(p, kAdjoint, rec_adj(p, i), p->g.dE, p->lam.p, p->lam.p, p->lam.p, nullptr),
+                               s));
     }
   }
-  // final accumulation at u_0 = 0 with weight 0.5 (pn_intensities.jl:402-404)
-  HIP_TRY(launch_dens(N, dims, dens_args(p, 0, p->ckpt.p, 0.5), s));
+  // A3(1) + the final accumulation at u_0 = 0 with weight 0.5 (pn_intensities.jl:402-404)
+  HIP_TRY(launch_rev(N, dims, rev_args(p, 0, p->ckpt.p, 0.5, true, false), s));
   return PNX_OK;
 }
 

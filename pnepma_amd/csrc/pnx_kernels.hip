@@ -7,36 +7,21 @@
 namespace pnx {
 
 hipError_t half_step_n1(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t dens_n1(int dims, const DensArgs& a, hipStream_t s);
 hipError_t half_step_n3(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t dens_n3(int dims, const DensArgs& a, hipStream_t s);
 hipError_t half_step_n5(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t dens_n5(int dims, const DensArgs& a, hipStream_t s);
 hipError_t half_step_n7(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t dens_n7(int dims, const DensArgs& a, hipStream_t s);
 hipError_t half_step_n9(int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t dens_n9(int dims, const DensArgs& a, hipStream_t s);
 
 hipError_t half_step_grouped_n1(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s);
-hipError_t dens_grouped_n1(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s);
 hipError_t half_step_grouped_n3(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s);
-hipError_t dens_grouped_n3(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s);
 hipError_t half_step_grouped_n5(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s);
-hipError_t dens_grouped_n5(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s);
 hipError_t half_step_grouped_n7(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s);
-hipError_t dens_grouped_n7(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s);
 hipError_t half_step_grouped_n9(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s);
-hipError_t dens_grouped_n9(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s);
 
 hipError_t half_step2_n1(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
 hipError_t half_step2_grouped_n1(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
@@ -133,17 +118,6 @@ hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* 
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s) {
-  switch (N) {
-    case 1: return dens_n1(dims, a, s);
-    case 3: return dens_n3(dims, a, s);
-    case 5: return dens_n5(dims, a, s);
-    case 7: return dens_n7(dims, a, s);
-    case 9: return dens_n9(dims, a, s);
-  }
-  return hipErrorInvalidValue;
-}
-
 hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
                                     const int2* items, const int* n_items, int grid, hipStream_t s) {
   switch (N) {
@@ -152,18 +126,6 @@ hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const
     case 5: return half_step_grouped_n5(dims, parity, mode, args, items, n_items, grid, s);
     case 7: return half_step_grouped_n7(dims, parity, mode, args, items, n_items, grid, s);
     case 9: return half_step_grouped_n9(dims, parity, mode, args, items, n_items, grid, s);
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                               hipStream_t s) {
-  switch (N) {
-    case 1: return dens_grouped_n1(dims, args, items, n_items, grid, s);
-    case 3: return dens_grouped_n3(dims, args, items, n_items, grid, s);
-    case 5: return dens_grouped_n5(dims, args, items, n_items, grid, s);
-    case 7: return dens_grouped_n7(dims, args, items, n_items, grid, s);
-    case 9: return dens_grouped_n9(dims, args, items, n_items, grid, s);
   }
   return hipErrorInvalidValue;
 }

@@ -91,9 +91,14 @@ struct DensArgs {
 };
 
 // Fused reverse pass of one reference-adjoint step (see rev_body).
+// Fused reverse pass of step i: [last odd adjoint half step of step i + 1]
+// (rec3 non-null), density pullback of step i, [first odd adjoint half step of
+// step i] (a1 != 0).  adj carries step i's adjoint record and dt = dE/2.
 struct RevArgs {
   HalfStepArgs adj;
   DensArgs dens;
+  const double* rec3;  // adjoint record of step i + 1, or null
+  int a1;
 };
 
 // Exact adjoint of one forward energy step, one fused pass per half step.
@@ -131,7 +136,6 @@ struct DiscArgs {
 
 // Launchers (defined in pnx_kernels.hip); return hipSuccess or the launch error.
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t launch_dens(int N, int dims, const DensArgs& a, hipStream_t s);
 // Two consecutive odd half steps in one pass (the neighbours of the even
 // classes do not change between them); see HalfStepArgs::rec2.
 hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, hipStream_t s);
@@ -140,8 +144,6 @@ hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, h
 // least *n_items.
 hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
                                     const int2* items, const int* n_items, int grid, hipStream_t s);
-hipError_t launch_dens_grouped(int N, int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                               hipStream_t s);
 hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
                                      const int* n_items, int grid, hipStream_t s);
 // Reverse pass of step i: density pullback of step i and the first odd

@@ -554,50 +554,17 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ r
   dens_core<N, DIM, C>(a, rec, int_fac, u_b, lam, i, j, k, p, beam);
 }
 
-template <int N, int DIM>
-__device__ __forceinline__ void dens_body(const DensArgs& a, const double* u, const double* rec, double int_fac,
-                                          int beam, int tile) {
-  const Layout& L = a.L;
-  int i, j, k;
-  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
-  const long p = i + (long)L.X * (j + (long)L.Y * k);
-  const long bo = (long)beam * L.beam_stride;
-  static_for<0, 8>([&](auto Cc) {
-    constexpr int C = decltype(Cc)::value;
-    if constexpr (class_active(C, DIM) && ncls<N, C>() > 0) {
-      const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
-                              (DIM >= 3 && odd_in(C, 2) && k == 0));
-      if (interior) dens_class<N, DIM, C>(a, glb(rec), int_fac, glb(u) + bo, glb(a.lam) + bo, i, j, k, p, beam);
-    }
-  });
-}
-
-template <int N, int DIM>
-__global__ __launch_bounds__(kBlock) void dens_kernel(DensArgs a) {
-  int beam, tile;
-  block_coords(a.L.n_tiles, a.n_beams, beam, tile);
-  dens_body<N, DIM>(a, a.u, a.rec, a.int_fac, beam, tile);
-}
-
-template <int N, int DIM>
-__global__ __launch_bounds__(kBlock) void dens_grouped(const DensArgs* __restrict__ args, const int2* __restrict__ items,
-                                                       const int* __restrict__ n_items) {
-  const int it = grouped_first();
-  if (it >= *n_items) return;
-  const int2 w = items[it];
-  const DensArgs& a = args[w.x];
-  dens_body<N, DIM>(a, a.u, a.rec, a.int_fac, 0, w.y);
-}
-
-// Fused reverse pass of step i (reference adjoint): the density pullback of
-// step i and the first odd adjoint half step of step i in one sweep.  The
-// pullback reads lambda only at its own point and the odd half step reads
-// only the even classes of lambda at neighbouring points, so each thread
-// carries its odd-class lambda from the one to the other in registers.  (Also
-// folding in the previous step's last odd half step saves more bytes, but the
-// kernel then needs every register of the SIMD and runs at occupancy 1,
-// which measured slower.)
+// Fused reverse pass of step i (reference adjoint, pn_intensities.jl:391-401):
+// the last odd adjoint half step of step i + 1, the density pullback of step
+// i and the first odd adjoint half step of step i in one sweep.  The odd half
+// steps read only the even classes of lambda, which none of the three
+// changes, at neighbouring points; the pullback reads lambda only at its own
+// point.  So each thread carries its odd-class lambda through the three in
+// registers, and the sweep reads and writes odd lambda once instead of three
+// times.  Even classes only take the pullback.
 //   a.adj:  from = to_src = to_dst = lambda; rec = adjoint record of step i, dt = dE/2
+//   a.rec3: adjoint record of step i + 1 (null for i = n)
+//   a.a1:   run the first odd half step of step i (0 for the final pullback at u_0)
 //   a.dens: u = u_i, rec = density record of step i, int_fac
 template <int N, int DIM, int C>
 __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam_b, gcd __restrict__ u_b, int i, int j,
@@ -609,8 +576,18 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
   double l0[NC], l1[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) l0[m] = lc[m * V + p];
+  if (a.rec3 != nullptr) {
+    update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.rec3), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+#pragma unroll
+    for (int m = 0; m < NC; ++m) l0[m] = l1[m];
+  }
   dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
-  update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+  if (a.a1) {
+    update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+  } else {
+#pragma unroll
+    for (int m = 0; m < NC; ++m) l1[m] = l0[m];
+  }
   gd out = glb(a.adj.to_dst) + (long)beam * L.beam_stride + L.cls_off[C];
 #pragma unroll
   for (int m = 0; m < NC; ++m) out[m * V + p] = l1[m];
@@ -680,49 +657,25 @@ __device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int tile
   });
 }
 
-template <int N>
-__device__ __forceinline__ void dens_body_cw(const DensArgs& a, const double* u, const double* rec, double int_fac,
-                                             int beam, int tile, int sub) {
-  const long bo = (long)beam * a.L.beam_stride;
-  class_per_wave_2d<N>(a.L, tile, sub, [&](auto Cc, int i, int j, long p) {
-    constexpr int C = decltype(Cc)::value;
-    dens_class<N, 2, C>(a, glb(rec), int_fac, glb(u) + bo, glb(a.lam) + bo, i, j, 0, p, beam);
-  });
-}
+// At P7 the fused reverse pass lands two registers above the three-waves-per-
+// SIMD limit (170 of 168); the bound keeps it at three.
+constexpr int rev_cw_min_blocks(int N) { return N <= 7 ? 3 : 1; }
 
 template <int N>
-__global__ __launch_bounds__(kBlock) void rev_kernel_cw(RevArgs a) {
+__global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(RevArgs a) {
   int beam, t4;
   block_coords(a.adj.L.n_tiles * 4, a.adj.n_beams, beam, t4);
   rev_body_cw<N>(a, beam, t4 >> 2, t4 & 3);
 }
 
 template <int N>
-__global__ __launch_bounds__(kBlock) void rev_grouped_cw(const RevArgs* __restrict__ args,
+__global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_grouped_cw(const RevArgs* __restrict__ args,
                                                          const int2* __restrict__ items,
                                                          const int* __restrict__ n_items) {
   const int it = grouped_first();
   if ((it >> 2) >= *n_items) return;
   const int2 w = items[it >> 2];
   rev_body_cw<N>(args[w.x], 0, w.y, it & 3);
-}
-
-template <int N>
-__global__ __launch_bounds__(kBlock) void dens_kernel_cw(DensArgs a) {
-  int beam, t4;
-  block_coords(a.L.n_tiles * 4, a.n_beams, beam, t4);
-  dens_body_cw<N>(a, a.u, a.rec, a.int_fac, beam, t4 >> 2, t4 & 3);
-}
-
-template <int N>
-__global__ __launch_bounds__(kBlock) void dens_grouped_cw(const DensArgs* __restrict__ args,
-                                                          const int2* __restrict__ items,
-                                                          const int* __restrict__ n_items) {
-  const int it = grouped_first();
-  if ((it >> 2) >= *n_items) return;
-  const int2 w = items[it >> 2];
-  const DensArgs& a = args[w.x];
-  dens_body_cw<N>(a, a.u, a.rec, a.int_fac, 0, w.y, it & 3);
 }
 
 template <int N, int DIM>
@@ -1048,30 +1001,6 @@ hipError_t rev_grouped_order(int dims, const RevArgs* args, const int2* items, c
     case 1: hipLaunchKernelGGL((rev_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
     case 2: hipLaunchKernelGGL((rev_grouped_cw<N>), dim3(grid * 4), dim3(kBlock), 0, s, args, items, n_items); break;
     case 3: hipLaunchKernelGGL((rev_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int N>
-hipError_t dens_grouped_order(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                              hipStream_t s) {
-  switch (dims) {
-    case 1: hipLaunchKernelGGL((dens_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 2: hipLaunchKernelGGL((dens_grouped_cw<N>), dim3(grid * 4), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 3: hipLaunchKernelGGL((dens_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int N>
-hipError_t dens_order(int dims, const DensArgs& a, hipStream_t s) {
-  const dim3 grid(a.L.n_tiles * a.n_beams), block(kBlock);
-  switch (dims) {
-    case 1: hipLaunchKernelGGL((dens_kernel<N, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((dens_kernel_cw<N>), dim3(grid.x * 4), block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((dens_kernel<N, 3>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

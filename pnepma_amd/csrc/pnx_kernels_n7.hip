@@ -7,8 +7,6 @@ hipError_t half_step_n7(int dims, int parity, int mode, const HalfStepArgs& a, h
   return half_step_order<7>(dims, parity, mode, a, s);
 }
 
-hipError_t dens_n7(int dims, const DensArgs& a, hipStream_t s) { return dens_order<7>(dims, a, s); }
-
 hipError_t rev_n7(int dims, const RevArgs& a, hipStream_t s) { return rev_order<7>(dims, a, s); }
 
 hipError_t rev_grouped_n7(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
@@ -28,11 +26,6 @@ hipError_t half_step2_grouped_n7(int dims, int mode, const HalfStepArgs* args, c
 hipError_t half_step_grouped_n7(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
                                  const int* n_items, int grid, hipStream_t s) {
   return half_step_grouped_order<7>(dims, parity, mode, args, items, n_items, grid, s);
-}
-
-hipError_t dens_grouped_n7(int dims, const DensArgs* args, const int2* items, const int* n_items, int grid,
-                            hipStream_t s) {
-  return dens_grouped_order<7>(dims, args, items, n_items, grid, s);
 }
 
 hipError_t disc_n7(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s) {

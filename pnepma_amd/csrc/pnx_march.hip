@@ -26,23 +26,20 @@ __device__ __forceinline__ void half_pass(const HalfStepArgs& a, const StepView&
   __syncthreads();
 }
 
+// Fused reverse pass at step i: [last odd adjoint half step of step i + 1],
+// density pullback with weight int_fac, [first odd adjoint half step of step i].
 template <int N>
-__device__ __forceinline__ void dens_pass(const DensArgs& a, const double* u, const double* rec, double int_fac) {
-  for (int t = 0; t < a.L.n_tiles; ++t) dens_body<N, 1>(a, u, rec, int_fac, 0, t);
-  __syncthreads();
-}
-
-// Fused density pullback + first odd adjoint half step of one reverse step.
-template <int N>
-__device__ __forceinline__ void rev_pass(const MarchArgs& m, const double* u, const double* rec_a, const double* rec_d) {
+__device__ __forceinline__ void rev_pass(const MarchArgs& m, int i, const double* u, double int_fac, bool a3, bool a1) {
   RevArgs ra;
   ra.adj = m.adj;
-  ra.adj.rec = rec_a;
+  ra.adj.rec = m.rec_a + (long)i * m.rec_len;
   ra.adj.dt = m.dens.dE / 2;
   ra.dens = m.dens;
   ra.dens.u = u;
-  ra.dens.rec = rec_d;
-  ra.dens.int_fac = 1.0;
+  ra.dens.rec = m.rec_d + (long)i * m.rec_len;
+  ra.dens.int_fac = int_fac;
+  ra.rec3 = a3 ? m.rec_a + (long)(i + 1) * m.rec_len : nullptr;
+  ra.a1 = a1 ? 1 : 0;
   for (int t = 0; t < ra.adj.L.n_tiles; ++t) rev_body<N, 1>(ra, 0, t);
   __syncthreads();
 }
@@ -87,11 +84,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
                               dE, L.dvol, m.I_out);
     } else if (o.kind == kOpReverse) {
       const double* rec = m.rec_a + (long)o.step * m.rec_len;
-      rev_pass<N>(m, src, rec, m.rec_d + (long)o.step * m.rec_len);
+      rev_pass<N>(m, o.step, src, 1.0, o.step < m.n, true);
       half_pass<N, 0, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, dE});
-      half_pass<N, 1, kAdjoint>(m.adj, StepView{m.lam, m.lam, m.lam, rec, nullptr, h});
     } else if (o.kind == kOpFinal) {
-      dens_pass<N>(m.dens, src, m.rec_d, 0.5);
+      rev_pass<N>(m, 0, src, 0.5, true, false);
     }
   }
 }

@@ -17,9 +17,11 @@
 // with the first odd half step only if it is the run's first (`first`), and
 // ends with the fused pair if it is not the run's last (`fuse`).
 //
-// A reverse step i runs as the density pullback of step i and the first odd
-// adjoint half step of step i in one fused pass, then the even and the last
-// odd adjoint half steps.
+// A reverse step i runs as one fused pass (the last odd adjoint half step of
+// step i + 1 when i < n, the density pullback of step i, the first odd adjoint
+// half step of step i), then the even adjoint half step of step i.  The final
+// operation fuses the last odd adjoint half step of step 1 with the pullback
+// at u_0.
 //
 // nck == 0 selects the taped sequence (the reference's own tape): every
 // forward state is kept in K = n segment slots.  The forward runs once, out of

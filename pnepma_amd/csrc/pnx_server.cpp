@@ -50,11 +50,11 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 constexpr size_t kAlign = 256;
-// step slots: fwd odd, fwd even, fwd odd, dens, (unused), adj even, adj odd,
-// fused fwd odd pair, fused dens + first adj odd
-constexpr int kSlots = 9;
-constexpr int kSlotPair = 7;
-constexpr int kSlotRev = 8;
+// Launch slots of a sub-tick, in launch order: the forward step's first odd
+// half step (first step of a run), even half step, last odd half step (last
+// step of a run) or fused odd pair; then the fused reverse pass and the even
+// adjoint half step.
+enum Slot : int { kSlotFwdOdd1, kSlotFwdEven, kSlotFwdOdd3, kSlotPair, kSlotRev, kSlotAdjEven, kSlots };
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 // Active requests that keep a lane's launches full: beyond these a request
@@ -499,7 +499,6 @@ double* buffer_of(const Request& r, int buf, int idx) {
 
 struct SlotWork {
   std::vector<HalfStepArgs> half;
-  std::vector<DensArgs> dens;
   std::vector<RevArgs> rev;
   std::vector<int2> items;
 };
@@ -524,11 +523,6 @@ void add_half(SlotWork& w, const HalfStepArgs& a) {
   w.half.push_back(a);
   for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
 }
-void add_dens(SlotWork& w, const DensArgs& a) {
-  const int idx = (int)w.dens.size();
-  w.dens.push_back(a);
-  for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
-}
 void add_rev(SlotWork& w, const RevArgs& a) {
   const int idx = (int)w.rev.size();
   w.rev.push_back(a);
@@ -551,8 +545,8 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       const double* rec = r.rec_f + (long)o.step * g.rec_len;
       // the recompute's intensity partials go to the scratch slot of step 0
       double* part = o.kind == kOpFwd ? r.partial + (long)o.step * g.partial_step : r.partial;
-      if (o.first) add_half(w[0], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
-      add_half(w[1], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part));
+      if (o.first) add_half(w[kSlotFwdOdd1], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
+      add_half(w[kSlotFwdEven], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part));
       if (o.fuse) {
         double* dst2 = buffer_of(r, o.dst2_buf, o.dst2_idx);
         HalfStepArgs a = make_half(s, r, kForward, rec, h, dst, dst, dst2 == dst ? nullptr : dst, nullptr);
@@ -561,25 +555,34 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
         a.to_dst2 = dst2;
         add_half(w[kSlotPair], a);
       } else {
-        add_half(w[2], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
+        add_half(w[kSlotFwdOdd3], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
       }
       if (o.reduce) plan.reduce.push_back(&r);
       break;
     }
     case kOpReverse: {
+      // A3(i+1) + pullback(i) + A1(i) fused, then AE(i)
       const double* rec = r.rec_a + (long)o.step * g.rec_len;
       RevArgs ra{};
       ra.adj = make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr);
       ra.dens = make_dens(s, r, o.step, src, 1.0);
+      ra.rec3 = o.step < r.d.n_steps ? rec + g.rec_len : nullptr;
+      ra.a1 = 1;
       add_rev(w[kSlotRev], ra);
-      add_half(w[5], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr));
-      add_half(w[6], make_half(s, r, kAdjoint, rec, h, r.lam, r.lam, r.lam, nullptr));
+      add_half(w[kSlotAdjEven], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr));
       break;
     }
-    case kOpFinal:
-      add_dens(w[3], make_dens(s, r, 0, src, 0.5));
+    case kOpFinal: {
+      // A3(1) + the pullback at u_0 with weight 0.5
+      RevArgs ra{};
+      ra.adj = make_half(s, r, kAdjoint, r.rec_a, h, r.lam, r.lam, r.lam, nullptr);
+      ra.dens = make_dens(s, r, 0, src, 0.5);
+      ra.rec3 = r.rec_a + g.rec_len;
+      ra.a1 = 0;
+      add_rev(w[kSlotRev], ra);
       whole.copy_out.push_back(&r);
       break;
+    }
     default:
       break;
   }
@@ -650,8 +653,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
         e.n_off = off;
         off = align_up(off + sizeof(int));
         e.args_off = off;
-        const size_t ab = sl == 3 ? w.dens.size() * sizeof(DensArgs)
-                                  : (sl >= kSlotRev ? w.rev.size() * sizeof(RevArgs) : w.half.size() * sizeof(HalfStepArgs));
+        const size_t ab = sl == kSlotRev ? w.rev.size() * sizeof(RevArgs) : w.half.size() * sizeof(HalfStepArgs);
         off = align_up(off + ab);
         e.items_off = off;
         off = align_up(off + w.items.size() * sizeof(int2));
@@ -668,9 +670,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
     }
     SlotWork& w = plan.sub[e.sub].groups[e.key][e.slot];
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
-    if (e.slot == 3)
-      std::memcpy(h + e.args_off, w.dens.data(), w.dens.size() * sizeof(DensArgs));
-    else if (e.slot >= kSlotRev)
+    if (e.slot == kSlotRev)
       std::memcpy(h + e.args_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
     else
       std::memcpy(h + e.args_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
@@ -689,10 +689,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       const int* nptr = reinterpret_cast<const int*>(ln.dev_args + e.n_off);
       const int2* items = reinterpret_cast<const int2*>(ln.dev_args + e.items_off);
       hipError_t err;
-      if (slot == 3) {
-        err = launch_dens_grouped(e.key.N, e.key.dims, reinterpret_cast<const DensArgs*>(ln.dev_args + e.args_off),
-                                  items, nptr, grid, st);
-      } else if (slot >= kSlotRev) {
+      if (slot == kSlotRev) {
         err = launch_rev_grouped(e.key.N, e.key.dims, reinterpret_cast<const RevArgs*>(ln.dev_args + e.args_off),
                                  items, nptr, grid, st);
       } else if (slot == kSlotPair) {
@@ -700,8 +697,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
                                         reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
                                         grid, st);
       } else {
-        const int mode = slot < 3 ? kForward : kAdjoint;
-        const int parity = (slot == 1 || slot == 5) ? 0 : 1;
+        const int mode = slot == kSlotAdjEven ? kAdjoint : kForward;
+        const int parity = (slot == kSlotFwdEven || slot == kSlotAdjEven) ? 0 : 1;
         err = launch_half_step_grouped(e.key.N, e.key.dims, parity, mode,
                                        reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
                                        grid, st);
@@ -717,7 +714,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       const long stn = r.g.L.beam_stride;
       if (launch_copy(r.ckpt + (long)cp.second * stn, r.state, stn, st) != hipSuccess) return -1;
     }
-    for (int sl : {0, 1, 2, kSlotPair})
+    for (int sl : {kSlotFwdOdd1, kSlotFwdEven, kSlotFwdOdd3, kSlotPair})
       if (launch_slot(t, sl)) return -1;
     for (Request* r : sp.reduce) {
       const Geometry& g = r->g;
@@ -726,7 +723,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
           hipSuccess)
         return -1;
     }
-    for (int sl : {3, kSlotRev, 5, 6})
+    for (int sl : {kSlotRev, kSlotAdjEven})
       if (launch_slot(t, sl)) return -1;
   }
   for (Request* r : plan.copy_out) {
