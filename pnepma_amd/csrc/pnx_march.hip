@@ -92,24 +92,30 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
   }
 }
 
-template <int N>
-__global__ __launch_bounds__(kBlock) void march_kernel(const MarchArgs* __restrict__ args, int n) {
-  for (int r = blockIdx.x; r < n; r += gridDim.x) march_body<N>(args[r]);
+// All PN orders in one launch: a tick of the 1D lane then lasts as long as
+// its slowest request, not the sum of one launch per order.  The kernel
+// allocates the registers of the largest order for every workgroup; the bound
+// of two workgroups per CU keeps it at 249 VGPRs without spills (unbounded it
+// took 256 VGPRs + 7 AGPRs, so each request held a whole CU).
+__global__ __launch_bounds__(kBlock, 2) void march_kernel(const MarchArgs* __restrict__ args, int n) {
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    switch (args[r].N) {
+      case 1: march_body<1>(args[r]); break;
+      case 3: march_body<3>(args[r]); break;
+      case 5: march_body<5>(args[r]); break;
+      case 7: march_body<7>(args[r]); break;
+      case 9: march_body<9>(args[r]); break;
+      default: break;
+    }
+  }
 }
 
 }  // namespace
 
-hipError_t launch_march(int N, const MarchArgs* args, int n_requests, hipStream_t s) {
+hipError_t launch_march(const MarchArgs* args, int n_requests, hipStream_t s) {
   if (n_requests <= 0) return hipSuccess;
   const dim3 grid(n_requests < 8192 ? n_requests : 8192), block(kBlock);
-  switch (N) {
-    case 1: hipLaunchKernelGGL(march_kernel<1>, grid, block, 0, s, args, n_requests); break;
-    case 3: hipLaunchKernelGGL(march_kernel<3>, grid, block, 0, s, args, n_requests); break;
-    case 5: hipLaunchKernelGGL(march_kernel<5>, grid, block, 0, s, args, n_requests); break;
-    case 7: hipLaunchKernelGGL(march_kernel<7>, grid, block, 0, s, args, n_requests); break;
-    case 9: hipLaunchKernelGGL(march_kernel<9>, grid, block, 0, s, args, n_requests); break;
-    default: return hipErrorInvalidValue;
-  }
+  hipLaunchKernelGGL(march_kernel, grid, block, 0, s, args, n_requests);
   return hipGetLastError();
 }
 

@@ -175,11 +175,12 @@ struct MarchArgs {
   long partial_step;
   int rec_len, sigma_off;
   int n, K, nck;
+  int N;              // PN order
   MarchPos pos;       // first operation of this launch
   int n_ops;          // operations to run in this launch
 };
 
-// Runs args[0..n_requests) (all of PN order N, 1D), one workgroup per request.
-hipError_t launch_march(int N, const MarchArgs* args, int n_requests, hipStream_t s);
+// Runs args[0..n_requests) (1D, any PN orders), one workgroup per request.
+hipError_t launch_march(const MarchArgs* args, int n_requests, hipStream_t s);
 
 }  // namespace pnx

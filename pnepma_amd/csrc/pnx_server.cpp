@@ -6,13 +6,12 @@
 // and hands each to a lane.  A lane owns one HIP stream and advances its
 // requests tick by tick, independently of the other lanes, so that the
 // kernels of different lanes overlap on the device:
-//   * 1D lane: one march launch per PN order per tick; a workgroup runs a
+//   * 1D lane: one march launch per tick for all PN orders; a workgroup runs a
 //     slice of up to kMarchOps operations of one request (march_next), with
 //     workgroup barriers between the half steps;
 //   * 2D/3D lanes (split by PN order): one operation per request per tick; for
-//     every (PN order, dimension) group and every one of the seven step slots
-//     one kernel runs over the work list of all requests in that slot (the
-//     grouped kernels).  kGridTicks ticks are planned and uploaded at once, so
+//     every (PN order, dimension) group and every step slot one kernel runs
+//     over the work list of all requests in that slot (the grouped kernels).  kGridTicks ticks are planned and uploaded at once, so
 //     the argument upload's latency is paid once per kGridTicks ticks.
 // Every request follows the operation sequence of pnx_forward_adjoint with
 // one beam (march_next), with the same device functions, so its results are
@@ -487,6 +486,7 @@ MarchArgs make_march(pnx_server* s, const Request& r) {
   m.n = r.d.n_steps;
   m.K = g.K;
   m.nck = g.nck;
+  m.N = r.d.N;
   return m;
 }
 
@@ -514,7 +514,7 @@ struct SubTick {
 // march slice per request (1D lane), with one argument upload.
 struct TickPlan {
   std::vector<SubTick> sub;                         // grid lanes
-  std::map<int, std::vector<MarchArgs>> march;      // 1D lane, per order
+  std::vector<MarchArgs> march;                     // 1D lane, all orders
   std::vector<Request*> copy_out;                   // results to copy back at the end
 };
 
@@ -596,7 +596,7 @@ void plan_march(pnx_server* s, Request& r, TickPlan& plan) {
   m.n_ops = std::min(kMarchOps, r.ops_left);
   for (int i = 0; i < m.n_ops; ++i) (void)march_next(r.pos, r.d.n_steps, r.g.K, r.g.nck);
   r.ops_left -= m.n_ops;
-  plan.march[r.d.N].push_back(m);
+  plan.march.push_back(m);
   if (r.ops_left == 0) plan.copy_out.push_back(&r);
   r.timing.ticks += 1;
 }
@@ -632,16 +632,16 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   struct Entry {
     int sub;   // sub-tick (grid lanes)
     GroupKey key;
-    int slot;  // -1: march block of order key.N
+    int slot;  // -1: the march block
     size_t n_off, args_off, items_off;
     int n_items;
   };
   std::vector<Entry> entries;
   size_t off = 0;
-  for (auto& kv : plan.march) {
-    Entry e{0, GroupKey{kv.first, 1}, -1, 0, 0, 0, (int)kv.second.size()};
+  if (!plan.march.empty()) {
+    Entry e{0, GroupKey{0, 1}, -1, 0, 0, 0, (int)plan.march.size()};
     e.args_off = off;
-    off = align_up(off + kv.second.size() * sizeof(MarchArgs));
+    off = align_up(off + plan.march.size() * sizeof(MarchArgs));
     entries.push_back(e);
   }
   for (int t = 0; t < (int)plan.sub.size(); ++t)
@@ -664,8 +664,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   char* h = ln.pinned[buf];
   for (const Entry& e : entries) {
     if (e.slot < 0) {
-      const auto& v = plan.march[e.key.N];
-      std::memcpy(h + e.args_off, v.data(), v.size() * sizeof(MarchArgs));
+      std::memcpy(h + e.args_off, plan.march.data(), plan.march.size() * sizeof(MarchArgs));
       continue;
     }
     SlotWork& w = plan.sub[e.sub].groups[e.key][e.slot];
@@ -679,8 +678,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   if (hipMemcpyAsync(ln.dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
   for (const Entry& e : entries)
     if (e.slot < 0 &&
-        launch_march(e.key.N, reinterpret_cast<const MarchArgs*>(ln.dev_args + e.args_off), e.n_items, st) !=
-            hipSuccess)
+        launch_march(reinterpret_cast<const MarchArgs*>(ln.dev_args + e.args_off), e.n_items, st) != hipSuccess)
       return -1;
   auto launch_slot = [&](int t, int slot) -> int {
     for (const Entry& e : entries) {
