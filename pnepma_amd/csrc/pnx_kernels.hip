@@ -24,42 +24,43 @@ hipError_t half_step_grouped_n9(int dims, int parity, int mode, const HalfStepAr
                                  const int* n_items, int grid, hipStream_t s);
 
 hipError_t half_step2_n1(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t half_step2_grouped_n1(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s);
 hipError_t half_step2_n3(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t half_step2_grouped_n3(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s);
 hipError_t half_step2_n5(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t half_step2_grouped_n5(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s);
 hipError_t half_step2_n7(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t half_step2_grouped_n7(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s);
 hipError_t half_step2_n9(int dims, int mode, const HalfStepArgs& a, hipStream_t s);
-hipError_t half_step2_grouped_n9(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s);
 
 hipError_t rev_n1(int dims, const RevArgs& a, hipStream_t s);
-hipError_t rev_grouped_n1(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s);
 hipError_t rev_n3(int dims, const RevArgs& a, hipStream_t s);
-hipError_t rev_grouped_n3(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s);
 hipError_t rev_n5(int dims, const RevArgs& a, hipStream_t s);
-hipError_t rev_grouped_n5(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s);
 hipError_t rev_n7(int dims, const RevArgs& a, hipStream_t s);
-hipError_t rev_grouped_n7(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s);
 hipError_t rev_n9(int dims, const RevArgs& a, hipStream_t s);
-hipError_t rev_grouped_n9(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s);
 
 hipError_t disc_n1(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n3(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n5(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n7(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 hipError_t disc_n9(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
+
+hipError_t phase_a_grouped_n1(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s);
+hipError_t phase_b_grouped_n1(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
+hipError_t phase_a_grouped_n3(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s);
+hipError_t phase_b_grouped_n3(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
+hipError_t phase_a_grouped_n5(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s);
+hipError_t phase_b_grouped_n5(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
+hipError_t phase_a_grouped_n7(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s);
+hipError_t phase_b_grouped_n7(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
+hipError_t phase_a_grouped_n9(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s);
+hipError_t phase_b_grouped_n9(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s);
 
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
   switch (N) {
@@ -83,18 +84,6 @@ hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, h
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
-                                     const int* n_items, int grid, hipStream_t s) {
-  switch (N) {
-    case 1: return half_step2_grouped_n1(dims, mode, args, items, n_items, grid, s);
-    case 3: return half_step2_grouped_n3(dims, mode, args, items, n_items, grid, s);
-    case 5: return half_step2_grouped_n5(dims, mode, args, items, n_items, grid, s);
-    case 7: return half_step2_grouped_n7(dims, mode, args, items, n_items, grid, s);
-    case 9: return half_step2_grouped_n9(dims, mode, args, items, n_items, grid, s);
-  }
-  return hipErrorInvalidValue;
-}
-
 hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s) {
   switch (N) {
     case 1: return rev_n1(dims, a, s);
@@ -106,14 +95,26 @@ hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                              hipStream_t s) {
+hipError_t launch_phase_a_grouped(int N, int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                                  const int* n_items, int grid, hipStream_t s) {
   switch (N) {
-    case 1: return rev_grouped_n1(dims, args, items, n_items, grid, s);
-    case 3: return rev_grouped_n3(dims, args, items, n_items, grid, s);
-    case 5: return rev_grouped_n5(dims, args, items, n_items, grid, s);
-    case 7: return rev_grouped_n7(dims, args, items, n_items, grid, s);
-    case 9: return rev_grouped_n9(dims, args, items, n_items, grid, s);
+    case 1: return phase_a_grouped_n1(dims, half, rev, items, n_items, grid, s);
+    case 3: return phase_a_grouped_n3(dims, half, rev, items, n_items, grid, s);
+    case 5: return phase_a_grouped_n5(dims, half, rev, items, n_items, grid, s);
+    case 7: return phase_a_grouped_n7(dims, half, rev, items, n_items, grid, s);
+    case 9: return phase_a_grouped_n9(dims, half, rev, items, n_items, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_phase_b_grouped(int N, int dims, const HalfStepArgs* half, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s) {
+  switch (N) {
+    case 1: return phase_b_grouped_n1(dims, half, items, n_items, grid, s);
+    case 3: return phase_b_grouped_n3(dims, half, items, n_items, grid, s);
+    case 5: return phase_b_grouped_n5(dims, half, items, n_items, grid, s);
+    case 7: return phase_b_grouped_n7(dims, half, items, n_items, grid, s);
+    case 9: return phase_b_grouped_n9(dims, half, items, n_items, grid, s);
   }
   return hipErrorInvalidValue;
 }

@@ -144,13 +144,22 @@ hipError_t launch_half_step2(int N, int dims, int mode, const HalfStepArgs& a, h
 // least *n_items.
 hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const HalfStepArgs* args,
                                     const int2* items, const int* n_items, int grid, hipStream_t s);
-hipError_t launch_half_step2_grouped(int N, int dims, int mode, const HalfStepArgs* args, const int2* items,
-                                     const int* n_items, int grid, hipStream_t s);
+// Phase launches of the server's grid lanes: one block per item of a device
+// work list whose entries carry their kind (item.x = kind << kItemKindShift |
+// argument index, item.y = tile; a 2D reverse item's tile is tile * 4 + the
+// two-row sub-tile of its block).
+//   phase A: kItemFwdEven (half[arg], forward even half step) and kItemRev (rev[arg]);
+//   phase B: kItemPair (half[arg], fused forward odd pair), kItemFwdOdd
+//            (forward odd half step) and kItemAdjEven (adjoint even half step).
+enum ItemKind : int { kItemFwdOdd = 0, kItemFwdEven = 1, kItemPair = 2, kItemRev = 3, kItemAdjEven = 4 };
+constexpr int kItemKindShift = 28;
+hipError_t launch_phase_a_grouped(int N, int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                                  const int* n_items, int grid, hipStream_t s);
+hipError_t launch_phase_b_grouped(int N, int dims, const HalfStepArgs* half, const int2* items, const int* n_items,
+                                  int grid, hipStream_t s);
 // Reverse pass of step i: density pullback of step i and the first odd
 // adjoint half step of step i in one sweep.
 hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s);
-hipError_t launch_rev_grouped(int N, int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                              hipStream_t s);
 // pb, pa in {-1 (none), 0 (even), 1 (odd)}
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 // Device material layer (pnx_material.hip): densities of every active class

@@ -457,17 +457,6 @@ __global__ __launch_bounds__(kBlock) void half_step2_kernel(HalfStepArgs a) {
   half_step2_body<N, DIM, MODE>(a, view_of(a), beam, tile);
 }
 
-template <int N, int DIM, int MODE>
-__global__ __launch_bounds__(kBlock) void half_step2_grouped(const HalfStepArgs* __restrict__ args,
-                                                             const int2* __restrict__ items,
-                                                             const int* __restrict__ n_items) {
-  const int it = grouped_first();
-  if (it >= *n_items) return;
-  const int2 w = items[it];
-  const HalfStepArgs& a = args[w.x];
-  half_step2_body<N, DIM, MODE>(a, view_of(a), 0, w.y);
-}
-
 // integrate_adjoint_densities! (pn_solver/pn_intensities.jl:129-187) for one
 // reverse step: du of every class without the exponential factor, then
 // rho_bar += lambda (s du + c u) dvol dE int_fac, plus the x-ray generation
@@ -668,14 +657,43 @@ __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(Re
   rev_body_cw<N>(a, beam, t4 >> 2, t4 & 3);
 }
 
-template <int N>
-__global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_grouped_cw(const RevArgs* __restrict__ args,
-                                                         const int2* __restrict__ items,
-                                                         const int* __restrict__ n_items) {
+// Phase kernels of the grid lanes: the kinds of one phase share a launch, so
+// a sub-tick is two launches over all its requests instead of one per kind.
+// The registers are those of the phase's largest kind.
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock, DIM == 2 ? rev_cw_min_blocks(N) : 1) void phase_a_grouped(
+    const HalfStepArgs* __restrict__ half, const RevArgs* __restrict__ rev, const int2* __restrict__ items,
+    const int* __restrict__ n_items) {
   const int it = grouped_first();
-  if ((it >> 2) >= *n_items) return;
-  const int2 w = items[it >> 2];
-  rev_body_cw<N>(args[w.x], 0, w.y, it & 3);
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const int kind = w.x >> kItemKindShift, arg = w.x & ((1 << kItemKindShift) - 1);
+  if (kind == kItemRev) {
+    if constexpr (DIM == 2)
+      rev_body_cw<N>(rev[arg], 0, w.y >> 2, w.y & 3);
+    else
+      rev_body<N, DIM>(rev[arg], 0, w.y);
+  } else {
+    const HalfStepArgs& a = half[arg];
+    half_step_body<N, DIM, 0, kForward>(a, view_of(a), 0, w.y);
+  }
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void phase_b_grouped(const HalfStepArgs* __restrict__ half,
+                                                          const int2* __restrict__ items,
+                                                          const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const int kind = w.x >> kItemKindShift, arg = w.x & ((1 << kItemKindShift) - 1);
+  const HalfStepArgs& a = half[arg];
+  if (kind == kItemPair)
+    half_step2_body<N, DIM, kForward>(a, view_of(a), 0, w.y);
+  else if (kind == kItemFwdOdd)
+    half_step_body<N, DIM, 1, kForward>(a, view_of(a), 0, w.y);
+  else
+    half_step_body<N, DIM, 0, kAdjoint>(a, view_of(a), 0, w.y);
 }
 
 template <int N, int DIM>
@@ -683,15 +701,6 @@ __global__ __launch_bounds__(kBlock) void rev_kernel(RevArgs a) {
   int beam, tile;
   block_coords(a.adj.L.n_tiles, a.adj.n_beams, beam, tile);
   rev_body<N, DIM>(a, beam, tile);
-}
-
-template <int N, int DIM>
-__global__ __launch_bounds__(kBlock) void rev_grouped(const RevArgs* __restrict__ args, const int2* __restrict__ items,
-                                                      const int* __restrict__ n_items) {
-  const int it = grouped_first();
-  if (it >= *n_items) return;
-  const int2 w = items[it];
-  rev_body<N, DIM>(args[w.x], 0, w.y);
 }
 
 // ----------------------------------------------------------- exact adjoint
@@ -968,21 +977,6 @@ hipError_t half_step2_order(int dims, int mode, const HalfStepArgs& a, hipStream
 }
 
 template <int N>
-hipError_t half_step2_grouped_order(int dims, int mode, const HalfStepArgs* args, const int2* items,
-                                    const int* n_items, int grid, hipStream_t s) {
-  switch (dims * 2 + mode) {
-#define PNX_CASE(D, M)                                                                                    \
-  case D * 2 + M:                                                                                         \
-    hipLaunchKernelGGL((half_step2_grouped<N, D, M>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); \
-    break;
-    PNX_CASE(1, 0) PNX_CASE(1, 1) PNX_CASE(2, 0) PNX_CASE(2, 1) PNX_CASE(3, 0) PNX_CASE(3, 1)
-#undef PNX_CASE
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <int N>
 hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
   const dim3 grid(a.adj.L.n_tiles * a.adj.n_beams), block(kBlock);
   switch (dims) {
@@ -995,12 +989,24 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
 }
 
 template <int N>
-hipError_t rev_grouped_order(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                             hipStream_t s) {
+hipError_t phase_a_grouped_order(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                                 const int* n_items, int grid, hipStream_t s) {
   switch (dims) {
-    case 1: hipLaunchKernelGGL((rev_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 2: hipLaunchKernelGGL((rev_grouped_cw<N>), dim3(grid * 4), dim3(kBlock), 0, s, args, items, n_items); break;
-    case 3: hipLaunchKernelGGL((rev_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, args, items, n_items); break;
+    case 1: hipLaunchKernelGGL((phase_a_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
+    case 2: hipLaunchKernelGGL((phase_a_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
+    case 3: hipLaunchKernelGGL((phase_a_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t phase_b_grouped_order(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                                 hipStream_t s) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((phase_b_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
+    case 2: hipLaunchKernelGGL((phase_b_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
+    case 3: hipLaunchKernelGGL((phase_b_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

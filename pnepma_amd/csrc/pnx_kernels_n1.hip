@@ -9,18 +9,8 @@ hipError_t half_step_n1(int dims, int parity, int mode, const HalfStepArgs& a, h
 
 hipError_t rev_n1(int dims, const RevArgs& a, hipStream_t s) { return rev_order<1>(dims, a, s); }
 
-hipError_t rev_grouped_n1(int dims, const RevArgs* args, const int2* items, const int* n_items, int grid,
-                           hipStream_t s) {
-  return rev_grouped_order<1>(dims, args, items, n_items, grid, s);
-}
-
 hipError_t half_step2_n1(int dims, int mode, const HalfStepArgs& a, hipStream_t s) {
   return half_step2_order<1>(dims, mode, a, s);
-}
-
-hipError_t half_step2_grouped_n1(int dims, int mode, const HalfStepArgs* args, const int2* items, const int* n_items,
-                                  int grid, hipStream_t s) {
-  return half_step2_grouped_order<1>(dims, mode, args, items, n_items, grid, s);
 }
 
 hipError_t half_step_grouped_n1(int dims, int parity, int mode, const HalfStepArgs* args, const int2* items,
@@ -30,6 +20,16 @@ hipError_t half_step_grouped_n1(int dims, int parity, int mode, const HalfStepAr
 
 hipError_t disc_n1(int dims, int pb, int pa, const DiscArgs& a, hipStream_t s) {
   return disc_order<1>(dims, pb, pa, a, s);
+}
+
+hipError_t phase_a_grouped_n1(int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
+                               const int* n_items, int grid, hipStream_t s) {
+  return phase_a_grouped_order<1>(dims, half, rev, items, n_items, grid, s);
+}
+
+hipError_t phase_b_grouped_n1(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                               hipStream_t s) {
+  return phase_b_grouped_order<1>(dims, half, items, n_items, grid, s);
 }
 
 }  // namespace pnx

@@ -10,8 +10,9 @@
 //     slice of up to kMarchOps operations of one request (march_next), with
 //     workgroup barriers between the half steps;
 //   * 2D/3D lanes (split by PN order): one operation per request per tick; for
-//     every (PN order, dimension) group and every step slot one kernel runs
-//     over the work list of all requests in that slot (the grouped kernels).  kGridTicks ticks are planned and uploaded at once, so
+//     every (PN order, dimension) group and every launch phase one kernel runs
+//     over the work list of all requests' passes in that phase (the grouped
+//     and phase kernels).  kGridTicks ticks are planned and uploaded at once, so
 //     the argument upload's latency is paid once per kGridTicks ticks.
 // Every request follows the operation sequence of pnx_forward_adjoint with
 // one beam (march_next), with the same device functions, so its results are
@@ -49,11 +50,13 @@ int set_error(int code, const std::string& msg);
 namespace {
 
 constexpr size_t kAlign = 256;
-// Launch slots of a sub-tick, in launch order: the forward step's first odd
-// half step (first step of a run), even half step, last odd half step (last
-// step of a run) or fused odd pair; then the fused reverse pass and the even
-// adjoint half step.
-enum Slot : int { kSlotFwdOdd1, kSlotFwdEven, kSlotFwdOdd3, kSlotPair, kSlotRev, kSlotAdjEven, kSlots };
+// Launch phases of a sub-tick, in launch order.  Each request's operation
+// puts one pass in each phase it uses, so the phases are its passes in order,
+// and requests in one phase launch are independent:
+//   first odd:  the forward step's first odd half step (first step of a run);
+//   phase A:    forward even half step | fused reverse pass;
+//   phase B:    fused odd pair or last odd half step | adjoint even half step.
+enum Phase : int { kPhaseFirstOdd, kPhaseA, kPhaseB, kPhases };
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 // Active requests that keep a lane's launches full: beyond these a request
@@ -497,7 +500,9 @@ double* buffer_of(const Request& r, int buf, int idx) {
   return r.state;
 }
 
-struct SlotWork {
+// One phase launch of one (order, dimension) group: argument blocks and the
+// work list (item.x = kind << kItemKindShift | argument index, item.y = tile).
+struct PhaseWork {
   std::vector<HalfStepArgs> half;
   std::vector<RevArgs> rev;
   std::vector<int2> items;
@@ -505,9 +510,9 @@ struct SlotWork {
 
 // One operation of every planned request of a grid lane.
 struct SubTick {
-  std::map<GroupKey, SlotWork[kSlots]> groups;
-  std::vector<std::pair<Request*, int>> copies;     // (request, checkpoint index) copies before the slots
-  std::vector<Request*> reduce;                     // forward finished: reduce intensities after slot 2
+  std::map<GroupKey, PhaseWork[kPhases]> groups;
+  std::vector<std::pair<Request*, int>> copies;     // (request, checkpoint index) copies before the phases
+  std::vector<Request*> reduce;                     // forward finished: reduce intensities after the phases
 };
 
 // What a lane enqueues at once: kGridTicks sub-ticks (grid lanes) or one
@@ -518,21 +523,23 @@ struct TickPlan {
   std::vector<Request*> copy_out;                   // results to copy back at the end
 };
 
-void add_half(SlotWork& w, const HalfStepArgs& a) {
+void add_half(PhaseWork& w, const HalfStepArgs& a, int kind) {
   const int idx = (int)w.half.size();
   w.half.push_back(a);
-  for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
+  for (int t = 0; t < a.L.n_tiles; ++t) w.items.push_back(make_int2(kind << kItemKindShift | idx, t));
 }
-void add_rev(SlotWork& w, const RevArgs& a) {
+void add_rev(PhaseWork& w, const RevArgs& a) {
   const int idx = (int)w.rev.size();
   w.rev.push_back(a);
-  for (int t = 0; t < a.adj.L.n_tiles; ++t) w.items.push_back(make_int2(idx, t));
+  // 2D: one class per wavefront, one block per two-row sub-tile
+  const int per_tile = a.adj.L.dims == 2 ? 4 : 1;
+  for (int t = 0; t < a.adj.L.n_tiles * per_tile; ++t) w.items.push_back(make_int2(kItemRev << kItemKindShift | idx, t));
 }
 
 // Plan the next operation of request r on a grid lane.
 void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
   const Geometry& g = r.g;
-  SlotWork* w = plan.groups[GroupKey{r.d.N, g.L.dims}];
+  PhaseWork* w = plan.groups[GroupKey{r.d.N, g.L.dims}];
   const double h = g.dE / 2;
   const MarchOp o = march_next(r.pos, r.d.n_steps, g.K, g.nck);
   r.ops_left -= 1;
@@ -545,17 +552,17 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       const double* rec = r.rec_f + (long)o.step * g.rec_len;
       // the recompute's intensity partials go to the scratch slot of step 0
       double* part = o.kind == kOpFwd ? r.partial + (long)o.step * g.partial_step : r.partial;
-      if (o.first) add_half(w[kSlotFwdOdd1], make_half(s, r, kForward, rec, h, src, src, dst, nullptr));
-      add_half(w[kSlotFwdEven], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part));
+      if (o.first) add_half(w[kPhaseFirstOdd], make_half(s, r, kForward, rec, h, src, src, dst, nullptr), kItemFwdOdd);
+      add_half(w[kPhaseA], make_half(s, r, kForward, rec, g.dE, dst, src, dst, part), kItemFwdEven);
       if (o.fuse) {
         double* dst2 = buffer_of(r, o.dst2_buf, o.dst2_idx);
         HalfStepArgs a = make_half(s, r, kForward, rec, h, dst, dst, dst2 == dst ? nullptr : dst, nullptr);
         a.rec2 = rec + g.rec_len;
         a.dt2 = h;
         a.to_dst2 = dst2;
-        add_half(w[kSlotPair], a);
+        add_half(w[kPhaseB], a, kItemPair);
       } else {
-        add_half(w[kSlotFwdOdd3], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr));
+        add_half(w[kPhaseB], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr), kItemFwdOdd);
       }
       if (o.reduce) plan.reduce.push_back(&r);
       break;
@@ -568,8 +575,8 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       ra.dens = make_dens(s, r, o.step, src, 1.0);
       ra.rec3 = o.step < r.d.n_steps ? rec + g.rec_len : nullptr;
       ra.a1 = 1;
-      add_rev(w[kSlotRev], ra);
-      add_half(w[kSlotAdjEven], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr));
+      add_rev(w[kPhaseA], ra);
+      add_half(w[kPhaseB], make_half(s, r, kAdjoint, rec, g.dE, r.lam, r.lam, r.lam, nullptr), kItemAdjEven);
       break;
     }
     case kOpFinal: {
@@ -579,7 +586,7 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
       ra.dens = make_dens(s, r, 0, src, 0.5);
       ra.rec3 = r.rec_a + g.rec_len;
       ra.a1 = 0;
-      add_rev(w[kSlotRev], ra);
+      add_rev(w[kPhaseA], ra);
       whole.copy_out.push_back(&r);
       break;
     }
@@ -630,31 +637,32 @@ int ensure_dev_args(Lane& ln, size_t bytes) {
 int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   hipStream_t st = ln.stream;
   struct Entry {
-    int sub;   // sub-tick (grid lanes)
+    int sub;    // sub-tick (grid lanes)
     GroupKey key;
-    int slot;  // -1: the march block
-    size_t n_off, args_off, items_off;
+    int phase;  // -1: the march block
+    size_t n_off, half_off, rev_off, items_off;
     int n_items;
   };
   std::vector<Entry> entries;
   size_t off = 0;
   if (!plan.march.empty()) {
-    Entry e{0, GroupKey{0, 1}, -1, 0, 0, 0, (int)plan.march.size()};
-    e.args_off = off;
+    Entry e{0, GroupKey{0, 1}, -1, 0, 0, 0, 0, (int)plan.march.size()};
+    e.half_off = off;
     off = align_up(off + plan.march.size() * sizeof(MarchArgs));
     entries.push_back(e);
   }
   for (int t = 0; t < (int)plan.sub.size(); ++t)
     for (auto& kv : plan.sub[t].groups)
-      for (int sl = 0; sl < kSlots; ++sl) {
-        SlotWork& w = kv.second[sl];
+      for (int ph = 0; ph < kPhases; ++ph) {
+        PhaseWork& w = kv.second[ph];
         if (w.items.empty()) continue;
-        Entry e{t, kv.first, sl, 0, 0, 0, (int)w.items.size()};
+        Entry e{t, kv.first, ph, 0, 0, 0, 0, (int)w.items.size()};
         e.n_off = off;
         off = align_up(off + sizeof(int));
-        e.args_off = off;
-        const size_t ab = sl == kSlotRev ? w.rev.size() * sizeof(RevArgs) : w.half.size() * sizeof(HalfStepArgs);
-        off = align_up(off + ab);
+        e.half_off = off;
+        off = align_up(off + w.half.size() * sizeof(HalfStepArgs));
+        e.rev_off = off;
+        off = align_up(off + w.rev.size() * sizeof(RevArgs));
         e.items_off = off;
         off = align_up(off + w.items.size() * sizeof(int2));
         entries.push_back(e);
@@ -663,44 +671,36 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   if (ensure_pinned(ln, buf, off) || ensure_dev_args(ln, off)) return -1;
   char* h = ln.pinned[buf];
   for (const Entry& e : entries) {
-    if (e.slot < 0) {
-      std::memcpy(h + e.args_off, plan.march.data(), plan.march.size() * sizeof(MarchArgs));
+    if (e.phase < 0) {
+      std::memcpy(h + e.half_off, plan.march.data(), plan.march.size() * sizeof(MarchArgs));
       continue;
     }
-    SlotWork& w = plan.sub[e.sub].groups[e.key][e.slot];
+    PhaseWork& w = plan.sub[e.sub].groups[e.key][e.phase];
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
-    if (e.slot == kSlotRev)
-      std::memcpy(h + e.args_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
-    else
-      std::memcpy(h + e.args_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
+    std::memcpy(h + e.half_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
+    std::memcpy(h + e.rev_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
     std::memcpy(h + e.items_off, w.items.data(), w.items.size() * sizeof(int2));
   }
   if (hipMemcpyAsync(ln.dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
   for (const Entry& e : entries)
-    if (e.slot < 0 &&
-        launch_march(reinterpret_cast<const MarchArgs*>(ln.dev_args + e.args_off), e.n_items, st) != hipSuccess)
+    if (e.phase < 0 &&
+        launch_march(reinterpret_cast<const MarchArgs*>(ln.dev_args + e.half_off), e.n_items, st) != hipSuccess)
       return -1;
-  auto launch_slot = [&](int t, int slot) -> int {
+  auto launch_phase = [&](int t, int phase) -> int {
     for (const Entry& e : entries) {
-      if (e.slot != slot || e.sub != t) continue;
+      if (e.phase != phase || e.sub != t) continue;
       const int grid = (e.n_items + 7) & ~7;  // one block per item, a multiple of 8 for the XCD mapping
       const int* nptr = reinterpret_cast<const int*>(ln.dev_args + e.n_off);
+      const auto* half = reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.half_off);
+      const auto* rev = reinterpret_cast<const RevArgs*>(ln.dev_args + e.rev_off);
       const int2* items = reinterpret_cast<const int2*>(ln.dev_args + e.items_off);
       hipError_t err;
-      if (slot == kSlotRev) {
-        err = launch_rev_grouped(e.key.N, e.key.dims, reinterpret_cast<const RevArgs*>(ln.dev_args + e.args_off),
-                                 items, nptr, grid, st);
-      } else if (slot == kSlotPair) {
-        err = launch_half_step2_grouped(e.key.N, e.key.dims, kForward,
-                                        reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
-                                        grid, st);
-      } else {
-        const int mode = slot == kSlotAdjEven ? kAdjoint : kForward;
-        const int parity = (slot == kSlotFwdEven || slot == kSlotAdjEven) ? 0 : 1;
-        err = launch_half_step_grouped(e.key.N, e.key.dims, parity, mode,
-                                       reinterpret_cast<const HalfStepArgs*>(ln.dev_args + e.args_off), items, nptr,
-                                       grid, st);
-      }
+      if (phase == kPhaseFirstOdd)  // kind kItemFwdOdd = 0: the items are (argument index, tile)
+        err = launch_half_step_grouped(e.key.N, e.key.dims, 1, kForward, half, items, nptr, grid, st);
+      else if (phase == kPhaseA)
+        err = launch_phase_a_grouped(e.key.N, e.key.dims, half, rev, items, nptr, grid, st);
+      else
+        err = launch_phase_b_grouped(e.key.N, e.key.dims, half, items, nptr, grid, st);
       if (err != hipSuccess) return -1;
     }
     return 0;
@@ -712,8 +712,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       const long stn = r.g.L.beam_stride;
       if (launch_copy(r.ckpt + (long)cp.second * stn, r.state, stn, st) != hipSuccess) return -1;
     }
-    for (int sl : {kSlotFwdOdd1, kSlotFwdEven, kSlotFwdOdd3, kSlotPair})
-      if (launch_slot(t, sl)) return -1;
+    for (int ph = 0; ph < kPhases; ++ph)
+      if (launch_phase(t, ph)) return -1;
     for (Request* r : sp.reduce) {
       const Geometry& g = r->g;
       if (launch_reduce_intensity(r->partial, g.partial_step, g.partial_beam, g.L.n_tiles * kWavesPerBlock, r->rec_f, g.rec_len,
@@ -721,8 +721,6 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
           hipSuccess)
         return -1;
     }
-    for (int sl : {kSlotRev, kSlotAdjEven})
-      if (launch_slot(t, sl)) return -1;
   }
   for (Request* r : plan.copy_out) {
     const long nI = r->d.n_lines, nrb = r->sz.rb, ngen = r->sz.gen;
