@@ -232,7 +232,7 @@ def main_serve(args):
     torch.zeros(1, device="cuda:0")
     from pnepma_amd import serving
     free, total = torch.cuda.mem_get_info()
-    budget = int(min(free * 0.8, 240e9))
+    budget = int(min(free * 0.9, 270e9))
     max_active = 4096
     # capacity probe on the head of the stream, all arriving at once; its tail
     # is poorly batched, so it underestimates the sustained capacity
@@ -280,7 +280,7 @@ def main_serve(args):
                                "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, CFL-0.5 step counts, "
                                "Poisson arrivals at 1.5x the all-at-once capacity of its first 512 requests",
                    "requests": len(preps), "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
-                   "tape_limit": "0.75 of the budget (server default)" if args.tape_fraction is None
+                   "tape_limit": "0.95 of the budget (server default)" if args.tape_fraction is None
                    else f"{args.tape_fraction} of the budget",
                    "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),

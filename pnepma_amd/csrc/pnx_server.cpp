@@ -944,7 +944,7 @@ int pnx_server_create(pnx_context* ctx, int64_t hbm_budget_bytes, int32_t max_ac
   auto* s = new pnx_server;
   s->ctx = ctx;
   s->budget = hbm_budget_bytes;
-  s->tape_limit = hbm_budget_bytes / 4 * 3;
+  s->tape_limit = hbm_budget_bytes / 20 * 19;
   s->max_active = max_active;
   if (s->arena.init((size_t)hbm_budget_bytes)) {
     release_server(s);
