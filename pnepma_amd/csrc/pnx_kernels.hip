@@ -62,6 +62,17 @@ hipError_t phase_a_grouped_n9(int dims, const HalfStepArgs* half, const RevArgs*
 hipError_t phase_b_grouped_n9(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
                                hipStream_t s);
 
+hipError_t pair_grouped_n1(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t pair_grouped_n3(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t pair_grouped_n5(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t pair_grouped_n7(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+hipError_t pair_grouped_n9(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s);
+
 hipError_t launch_half_step(int N, int dims, int parity, int mode, const HalfStepArgs& a, hipStream_t s) {
   switch (N) {
     case 1: return half_step_n1(dims, parity, mode, a, s);
@@ -115,6 +126,18 @@ hipError_t launch_phase_b_grouped(int N, int dims, const HalfStepArgs* half, con
     case 5: return phase_b_grouped_n5(dims, half, items, n_items, grid, s);
     case 7: return phase_b_grouped_n7(dims, half, items, n_items, grid, s);
     case 9: return phase_b_grouped_n9(dims, half, items, n_items, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pair_grouped(int N, int dims, const HalfStepArgs* half, const int2* items, const int* n_items,
+                               int grid, hipStream_t s) {
+  switch (N) {
+    case 1: return pair_grouped_n1(dims, half, items, n_items, grid, s);
+    case 3: return pair_grouped_n3(dims, half, items, n_items, grid, s);
+    case 5: return pair_grouped_n5(dims, half, items, n_items, grid, s);
+    case 7: return pair_grouped_n7(dims, half, items, n_items, grid, s);
+    case 9: return pair_grouped_n9(dims, half, items, n_items, grid, s);
   }
   return hipErrorInvalidValue;
 }

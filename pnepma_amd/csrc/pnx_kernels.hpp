@@ -149,12 +149,14 @@ hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const
 // argument index, item.y = tile; a 2D reverse item's tile is tile * 4 + the
 // two-row sub-tile of its block).
 //   phase A: kItemFwdEven (half[arg], forward even half step) and kItemRev (rev[arg]);
-//   phase B: kItemPair (half[arg], fused forward odd pair), kItemFwdOdd
-//            (forward odd half step) and kItemAdjEven (adjoint even half step).
+//   pair:    kItemPair (half[arg], fused forward odd pair);
+//   phase B: kItemFwdOdd (forward odd half step) and kItemAdjEven (adjoint even half step).
 enum ItemKind : int { kItemFwdOdd = 0, kItemFwdEven = 1, kItemPair = 2, kItemRev = 3, kItemAdjEven = 4 };
 constexpr int kItemKindShift = 28;
 hipError_t launch_phase_a_grouped(int N, int dims, const HalfStepArgs* half, const RevArgs* rev, const int2* items,
                                   const int* n_items, int grid, hipStream_t s);
+hipError_t launch_pair_grouped(int N, int dims, const HalfStepArgs* half, const int2* items, const int* n_items,
+                               int grid, hipStream_t s);
 hipError_t launch_phase_b_grouped(int N, int dims, const HalfStepArgs* half, const int2* items, const int* n_items,
                                   int grid, hipStream_t s);
 // Reverse pass of step i: density pullback of step i and the first odd

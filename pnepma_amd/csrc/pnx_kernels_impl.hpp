@@ -181,6 +181,63 @@ __device__ __forceinline__ StepView view_of(const HalfStepArgs& a) {
   return StepView{a.from, a.to_src, a.to_dst, a.rec, a.partial, a.dt, a.rec2, a.to_dst2, a.dt2};
 }
 
+// Change of the stencil result du of class C when its own values go from u to
+// u1 and the source factor from gfac to gfac1 (the second update of a fused
+// odd pair).  The neighbours of the other parity are the same for both
+// updates, so only the boundary ghosts change, and only at boundary cells:
+// a left ghost 2(-B u) - v_R moves v_R - v_L by 2 B (u1 - u), a right ghost
+// 2(B u + g) - v_L by 2 (B (u1 - u) + g1 - g).
+template <int N, int DIM, int C, bool SRC>
+__device__ __forceinline__ void stencil_ghost_delta(const Layout& L, gcd __restrict__ coef, const double* u,
+                                                    const double* u1, double* du, int i, int j, int k, double gfac,
+                                                    double gfac1, gcd __restrict__ prof_b, gcd __restrict__ mms) {
+  constexpr int NC = ncls<N, C>();
+  static_for<0, DIM>([&](auto Dc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr int F = switch_axis(C, D);
+    constexpr int NF = ncls<N, F>();
+    if constexpr (NF > 0 && odd_in(C, D) == 0) {
+      const int idx = D == 0 ? i : (D == 1 ? j : k);
+      const int M = D == 0 ? L.Mx : (D == 1 ? L.My : L.Mz);
+      const bool left = idx == 0, right = idx == M - 1;
+      if (left || right) {
+        constexpr int boff = Tab<N>::t.b_off[F][D];
+        gcd __restrict__ Bm = coef + boff;
+        double pr = 0.0;
+        if constexpr (SRC && D == 0) pr = prof_b[F * (long)L.Y * L.Z + j + (long)L.Y * k];
+        const double ih = L.inv_h[D];
+        double dd[NF];
+#pragma unroll
+        for (int J = 0; J < NF; ++J) {
+          double dbc = 0.0;
+#pragma unroll
+          for (int I = 0; I < NC; ++I) dbc += Bm[J * NC + I] * (u1[I] - u[I]);
+          double dg = 0.0;
+          if constexpr (SRC && D == 0)
+            if (!left) dg = mms[F * kMmsStride + J] * pr * (gfac1 - gfac);
+          dd[J] = 2.0 * (dbc + dg) * ih;
+        }
+        static_for<0, NF>([&](auto Jc) {
+          constexpr int J = decltype(Jc)::value;
+          static_for<0, NC>([&](auto Ic) {
+            constexpr int I = decltype(Ic)::value;
+            if constexpr (Tab<N>::t.nz[C][D][I][J] >= 0) {
+              constexpr double av = SRC ? Tab<N>::t.a[C][D][I][J] : -Tab<N>::t.a[C][D][I][J];
+              du[I] += av * dd[J];
+            }
+          });
+        });
+      }
+    }
+  });
+}
+
+// The update of class C at point p from its stencil result du (the rest of
+// right_hand_side! + update!): un = u + dt * phi * (q - du - C u) / S.
+template <int N, int C, int MODE>
+__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+                                             const double* u, const double* du, double* un, long p, int beam);
+
 // One update of class C at point p (right_hand_side! + update!,
 // solver_core.jl:92-106, solver.jl:122-124): own values u -> un, in registers.
 template <int N, int DIM, int C, int MODE>
@@ -188,15 +245,23 @@ __device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restric
                                             gcd __restrict__ from, const double* u, double* un, int i, int j, int k,
                                             long p, int beam) {
   constexpr int NC = ncls<N, C>();
-  const Layout& L = a.L;
-  const long V = L.V;
-  const int ne = L.n_e;
+  const int ne = a.L.n_e;
   double du[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) du[m] = 0.0;
   const double gfac = rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, MODE == kForward>(L, from, glb(a.coef), u, du, i, j, k, p, gfac,
+  stencil<N, DIM, C, MODE == kForward>(a.L, from, glb(a.coef), u, du, i, j, k, p, gfac,
                                        glb(a.prof) + beam * a.prof_stride, glb(a.mms));
+  apply_update<N, C, MODE>(a, rec, dt, u, du, un, p, beam);
+}
+
+template <int N, int C, int MODE>
+__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+                                             const double* u, const double* du, double* un, long p, int beam) {
+  constexpr int NC = ncls<N, C>();
+  const Layout& L = a.L;
+  const long V = L.V;
+  const int ne = L.n_e;
   double S, Cl[N + 1];
   mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
   double phi[N + 1];
@@ -268,19 +333,31 @@ template <int N, int DIM, int C, int MODE>
 __device__ __forceinline__ void update_class2(const HalfStepArgs& a, const StepView& v, gcd __restrict__ from, gcd src,
                                               gd dst, gd dst2, int i, int j, int k, long p, int beam) {
   constexpr int NC = ncls<N, C>();
+  constexpr bool SRC = MODE == kForward;
   const Layout& L = a.L;
   const long V = L.V;
+  const int ne = L.n_e;
   gcd uc = src + L.cls_off[C];
-  double u[NC], u1[NC], u2[NC];
+  double u[NC], u1[NC], u2[NC], du[NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) u[m] = uc[m * V + p];
-  update_core<N, DIM, C, MODE>(a, glb(v.rec), v.dt, from, u, u1, i, j, k, p, beam);
+  for (int m = 0; m < NC; ++m) {
+    u[m] = uc[m * V + p];
+    du[m] = 0.0;
+  }
+  gcd __restrict__ rec = glb(v.rec);
+  gcd __restrict__ rec2 = glb(v.rec2);
+  const double gfac = rec[ne + ne * (N + 1)], gfac2 = rec2[ne + ne * (N + 1)];
+  gcd __restrict__ prof_b = glb(a.prof) + beam * a.prof_stride;
+  stencil<N, DIM, C, SRC>(L, from, glb(a.coef), u, du, i, j, k, p, gfac, prof_b, glb(a.mms));
+  apply_update<N, C, MODE>(a, rec, v.dt, u, du, u1, p, beam);
+  // the second update's stencil differs from the first only in its boundary ghosts
+  stencil_ghost_delta<N, DIM, C, SRC>(L, glb(a.coef), u, u1, du, i, j, k, gfac, gfac2, prof_b, glb(a.mms));
+  apply_update<N, C, MODE>(a, rec2, v.dt2, u1, du, u2, p, beam);
   if (dst) {
     gd out = dst + L.cls_off[C];
 #pragma unroll
     for (int m = 0; m < NC; ++m) out[m * V + p] = u1[m];
   }
-  update_core<N, DIM, C, MODE>(a, glb(v.rec2), v.dt2, from, u1, u2, i, j, k, p, beam);
   gd out2 = dst2 + L.cls_off[C];
 #pragma unroll
   for (int m = 0; m < NC; ++m) out2[m * V + p] = u2[m];
@@ -679,6 +756,20 @@ __global__ __launch_bounds__(kBlock, DIM == 2 ? rev_cw_min_blocks(N) : 1) void p
   }
 }
 
+// The fused odd pair has its own launch: in one kernel with the two kinds of
+// phase B the compiler allocated more registers than any of them needs (2D
+// P7: 209 VGPRs, P9: 255 and occupancy 1; bounded, it spilled).
+template <int N, int DIM>
+__global__ __launch_bounds__(kBlock) void pair_grouped(const HalfStepArgs* __restrict__ half,
+                                                       const int2* __restrict__ items,
+                                                       const int* __restrict__ n_items) {
+  const int it = grouped_first();
+  if (it >= *n_items) return;
+  const int2 w = items[it];
+  const HalfStepArgs& a = half[w.x & ((1 << kItemKindShift) - 1)];
+  half_step2_body<N, DIM, kForward>(a, view_of(a), 0, w.y);
+}
+
 template <int N, int DIM>
 __global__ __launch_bounds__(kBlock) void phase_b_grouped(const HalfStepArgs* __restrict__ half,
                                                           const int2* __restrict__ items,
@@ -688,9 +779,7 @@ __global__ __launch_bounds__(kBlock) void phase_b_grouped(const HalfStepArgs* __
   const int2 w = items[it];
   const int kind = w.x >> kItemKindShift, arg = w.x & ((1 << kItemKindShift) - 1);
   const HalfStepArgs& a = half[arg];
-  if (kind == kItemPair)
-    half_step2_body<N, DIM, kForward>(a, view_of(a), 0, w.y);
-  else if (kind == kItemFwdOdd)
+  if (kind == kItemFwdOdd)
     half_step_body<N, DIM, 1, kForward>(a, view_of(a), 0, w.y);
   else
     half_step_body<N, DIM, 0, kAdjoint>(a, view_of(a), 0, w.y);
@@ -995,6 +1084,18 @@ hipError_t phase_a_grouped_order(int dims, const HalfStepArgs* half, const RevAr
     case 1: hipLaunchKernelGGL((phase_a_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
     case 2: hipLaunchKernelGGL((phase_a_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
     case 3: hipLaunchKernelGGL((phase_a_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, half, rev, items, n_items); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int N>
+hipError_t pair_grouped_order(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                              hipStream_t s) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL((pair_grouped<N, 1>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
+    case 2: hipLaunchKernelGGL((pair_grouped<N, 2>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
+    case 3: hipLaunchKernelGGL((pair_grouped<N, 3>), dim3(grid), dim3(kBlock), 0, s, half, items, n_items); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

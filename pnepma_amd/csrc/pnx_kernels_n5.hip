@@ -32,4 +32,9 @@ hipError_t phase_b_grouped_n5(int dims, const HalfStepArgs* half, const int2* it
   return phase_b_grouped_order<5>(dims, half, items, n_items, grid, s);
 }
 
+hipError_t pair_grouped_n5(int dims, const HalfStepArgs* half, const int2* items, const int* n_items, int grid,
+                            hipStream_t s) {
+  return pair_grouped_order<5>(dims, half, items, n_items, grid, s);
+}
+
 }  // namespace pnx

@@ -55,8 +55,10 @@ constexpr size_t kAlign = 256;
 // and requests in one phase launch are independent:
 //   first odd:  the forward step's first odd half step (first step of a run);
 //   phase A:    forward even half step | fused reverse pass;
-//   phase B:    fused odd pair or last odd half step | adjoint even half step.
-enum Phase : int { kPhaseFirstOdd, kPhaseA, kPhaseB, kPhases };
+//   pair:       fused odd pair (a step that is not the last of its run);
+//   phase B:    last odd half step of a run | adjoint even half step.
+// (The pair and phase B are independent of each other.)
+enum Phase : int { kPhaseFirstOdd, kPhaseA, kPhasePair, kPhaseB, kPhases };
 constexpr int kMarchOps = 48;  // operations per 1D request per march launch
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 // Active requests that keep a lane's launches full: beyond these a request
@@ -560,7 +562,7 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
         a.rec2 = rec + g.rec_len;
         a.dt2 = h;
         a.to_dst2 = dst2;
-        add_half(w[kPhaseB], a, kItemPair);
+        add_half(w[kPhasePair], a, kItemPair);
       } else {
         add_half(w[kPhaseB], make_half(s, r, kForward, rec, h, dst, dst, dst, nullptr), kItemFwdOdd);
       }
@@ -699,6 +701,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
         err = launch_half_step_grouped(e.key.N, e.key.dims, 1, kForward, half, items, nptr, grid, st);
       else if (phase == kPhaseA)
         err = launch_phase_a_grouped(e.key.N, e.key.dims, half, rev, items, nptr, grid, st);
+      else if (phase == kPhasePair)
+        err = launch_pair_grouped(e.key.N, e.key.dims, half, items, nptr, grid, st);
       else
         err = launch_phase_b_grouped(e.key.N, e.key.dims, half, items, nptr, grid, st);
       if (err != hipSuccess) return -1;
