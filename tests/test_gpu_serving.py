@@ -83,3 +83,48 @@ def test_served_matches_oracle(oracle_lib, taped):
         for c in spec.problem.gspec.active_classes():
             assert rel_err(pr.rho_bar[c], rb[c]) < 1e-8
         assert rel_err(pr.si.noa[:, pr.si.el_idx] * pr.gen, acb) < 1e-8
+
+
+def _spec_3d(N, M, n_steps, seed):
+    from pnepma_amd import workloads
+    from pnepma_amd.grid import GridSpec
+    from pnepma_amd.material import CellFractions, MassFractionMaterial
+    from pnepma_amd.physics import XRay, element
+    from pnepma_amd.problem import ForwardPNProblem
+    rng = np.random.default_rng(seed)
+    names = ["Fe", "Cr", "Ni"]
+    els = [element(s) for s in names]
+    g = GridSpec.make(M, (-1.5e-4, 0.0), (-0.5e-4, 0.5e-4), (-0.4e-4, 0.4e-4))
+    p = ForwardPNProblem(N=N, gspec=g, elements=els, beam_energy=(15e3, workloads.SIGMA_E),
+                         beam_positions=[(0.0, 0.0, 0.0)], beam_sigma=50 * workloads.NM,
+                         beam_direction=((-1.0, 0.0, 0.0), workloads.KAPPA), E_initial=15.5e3, E_cutoff=5e3,
+                         n_steps=n_steps, source_mode="lateral")
+    p.x_rays = [XRay(s) for s in names]
+    vals = workloads.inclusion_fractions(tuple(M), len(els), 4, rng)
+    mat = MassFractionMaterial(els, CellFractions(vals, *[workloads.cell_edges(a) for a in g.axes],
+                                                  embedding=vals.reshape(-1, len(els))[0]))
+    return serving.Spec(p, mat, workloads.detector_40deg(), rng.uniform(0.5, 1.5, len(els)))
+
+
+@pytest.mark.parametrize("tape", [True, False])
+def test_served_3d_equals_alone(tape):
+    """3D requests (the grid lanes' DIM = 3 kernels) mixed with 2D ones."""
+    specs = [_spec_3d(3, (10, 8, 6), 24, 1), _spec_3d(5, (8, 6, 6), 20, 2), _spec_3d(3, (12, 6, 5), 18, 3)]
+    specs += _specs(n=4, seed=13)
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=64)
+    if not tape:
+        srv.set_tape_limit(0)
+    tickets = [srv.submit(pr, arrival_s=0.001 * n) for n, pr in enumerate(preps)]
+    for t in tickets:
+        tm = srv.wait(t)
+        assert tm.status == 0 and tm.taped == int(tape)
+    srv.close()
+    for spec, pr in zip(specs, preps):
+        I, rb, gen = _alone(spec)
+        p = spec.problem
+        print(f"N={p.N} M={p.gspec.M} steps={p.n_steps} I={pr.I_out}", flush=True)
+        assert rel_err(pr.I_out, I) < 1e-12
+        for c in p.gspec.active_classes():
+            assert rel_err(pr.rho_bar[c], rb[c]) < 1e-12
+        assert rel_err(pr.gen, gen) < 1e-12
