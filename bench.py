@@ -84,16 +84,25 @@ def roofline_probe(reps=50):
     from pnepma_amd.solver import DeviceProblem
     import numpy as np
     p, mat, stds, det = workloads.raster2d(n_steps=400, n_beams=PROBE_BEAMS)
-    dp = DeviceProblem(p, solve_inputs(p, mat, det), adjoint=False)
+    si = solve_inputs(p, mat, det)
+    dp = DeviceProblem(p, si, adjoint=False)
     ms, by = dp.half_step_bench(0, reps)
     achieved = by / (ms * 1e-3) / 1e9
     dp.close()
+    # the fused reverse pass (the largest kernel of the served stream) on the same batch
+    dp = DeviceProblem(p, si, ckpt_every=20)
+    ms_r, by_r = dp.rev_bench(reps)
+    dp.close()
+    achieved_r = by_r / (ms_r * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
             "kernel": f"half_step_kernel<7,2,0,fwd> (even parity, 2D P7 128x96, {PROBE_BEAMS} beams)",
             "kernel_ms": ms, "algorithmic_bytes": by,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --workload kernel, "
-                              "calibrated on a 1 GiB copy (profiles/half_step_p7_2d_128x96_160beams.md)"}
+                              "calibrated on a 1 GiB copy (profiles/half_step_p7_2d_128x96_160beams.md)",
+            "reverse": {"kernel": f"rev_kernel_cw<7> (fused reverse pass, 2D P7 128x96, {PROBE_BEAMS} beams)",
+                        "achieved": achieved_r, "frac": achieved_r / HBM_PEAK_GBS, "kernel_ms": ms_r,
+                        "algorithmic_bytes": by_r, "bound": "latency (occupancy 3)"}}
 
 
 def serve_run(preps, rate, budget, max_active, tape_fraction=None):
@@ -359,8 +368,8 @@ def main_kernel(args):
     roof = roofline_probe()
     out = {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": 0, "warmup": 0,
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic", "config": {"workload": f"kernel probe: fused half step of 2D P7 128x96, "
-                                                      f"{PROBE_BEAMS} beams, plus 5 copies of 1 GiB"},
+           "data": "synthetic", "config": {"workload": f"kernel probe: fused even half step and fused reverse pass of "
+                                                      f"2D P7 128x96, {PROBE_BEAMS} beams, plus 5 copies of 1 GiB"},
            "roofline": roof, "copy_bytes": 2 * x.numel() * 8}
     print(json.dumps(out), flush=True)
 

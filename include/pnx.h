@@ -173,6 +173,11 @@ int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out);
  * back-to-back launches on the problem's stream, and the algorithmic bytes
  * of one such launch. */
 int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, double* bytes_out);
+/* The same for the fused reverse pass of the reference adjoint (the last odd
+ * adjoint half step of step 2, the density pullback of step 1 and the first
+ * odd adjoint half step of step 1) on the problem's forward state; needs the
+ * adjoint records and at least two energy steps. */
+int pnx_rev_bench(pnx_problem* p, int reps, double* ms_out, double* bytes_out);
 
 /* ------------------------------------------------------------------ serving
  * A continuous-batching server of forward+adjoint requests with mixed PN

@@ -58,6 +58,7 @@ SIGNATURES = {
     "pnx_forward_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pnx_forward_adjoint_timed": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "pnx_half_step_bench": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pnx_rev_bench": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pnx_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
     "pnx_server_destroy": (C.c_int, [C.c_void_p]),
     "pnx_server_clock": (C.c_double, [C.c_void_p]),

@@ -877,6 +877,52 @@ int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out) {
   return PNX_OK;
 }
 
+int pnx_rev_bench(pnx_problem* p, int reps, double* ms_out, double* bytes_out) {
+  int rc = check_ready(p, true);
+  if (rc) return rc;
+  if (reps < 1 || p->d.n_steps < 2) return fail(PNX_ERR_ARG, "bad argument");
+  const Layout& L = p->L;
+  // algorithmic bytes: u and lambda of every class read once, odd lambda
+  // written, densities read and rho_bar read and written at every class
+  // point, generation sums read and written at eee points
+  double bytes = 0;
+  for (int c = 0; c < 8; ++c) {
+    if (!(class_active(c, L.dims) && p->T.n[c] > 0)) continue;
+    double np = 1;
+    for (int a = 0; a < L.dims; ++a) np *= odd_in(c, a) ? p->d.M[a] - 1 : p->d.M[a];
+    bytes += np * p->T.n[c] * 8.0 * (parity_of(c) == 1 ? 3 : 2);
+    bytes += np * p->d.n_elements * 24.0;
+    if (c == 0) bytes += np * p->d.n_lines * 16.0;
+  }
+  bytes *= p->d.n_beams;
+  if (bytes_out) *bytes_out = bytes;
+  if (!ms_out) return PNX_OK;
+  hipStream_t s = p->ctx->stream;
+  // lambda, rho_bar, the generation sums and the seed are allocated by the
+  // first adjoint solve; a problem that has not had one gets them here
+  if ((rc = alloc_adjoint(p))) return rc;
+  HIP_TRY(launch_fill(p->lam.p, (long)p->lam.n, 0.0, s));
+  HIP_TRY(launch_fill(p->rho_bar_b.p, (long)p->rho_bar_b.n, 0.0, s));
+  HIP_TRY(launch_fill(p->gen_b.p, (long)p->gen_b.n, 0.0, s));
+  HIP_TRY(launch_fill(p->ibar.p, (long)p->ibar.n, 0.0, s));
+  // step 1 with the last odd half step of step 2 folded in, on the forward state
+  const RevArgs ra = rev_args(p, 1, p->state.p, 1.0, true, true);
+  HIP_TRY(launch_rev(p->d.N, L.dims, ra, s));  // warm-up
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) HIP_TRY(launch_rev(p->d.N, L.dims, ra, s));
+  HIP_TRY(hipEventRecord(e1, s));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms / reps;
+  return PNX_OK;
+}
+
 int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, double* bytes_out) {
   int rc = check_ready(p, false);
   if (rc) return rc;

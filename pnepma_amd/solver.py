@@ -228,6 +228,11 @@ class DeviceProblem:
         check(lib().pnx_half_step_bench(self.handle, parity, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def rev_bench(self, reps: int):
+        ms, by = C.c_double(), C.c_double()
+        check(lib().pnx_rev_bench(self.handle, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
 
 def calc_intensities(problem: ForwardPNProblem, material, detector, ctx: Context | None = None) -> np.ndarray:
     """Intensities of every beam, shape (n_beams, n_lines)."""

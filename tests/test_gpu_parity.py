@@ -126,3 +126,19 @@ def test_kratios_slab1d_workload(oracle_lib):
     k, k_ref = I / Is, I_ref / Is_ref
     print("k gpu", k, "k ref", k_ref, flush=True)
     assert np.max(np.abs(k - k_ref) / np.abs(k_ref)) < 1e-10
+
+
+def test_rev_bench_on_a_fresh_problem():
+    """The reverse-pass timing entry point allocates the adjoint buffers itself
+    (a problem that has had no adjoint solve has none)."""
+    from pnepma_amd import workloads
+    from pnepma_amd.problem import solve_inputs
+    p, mat, stds, det = workloads.raster2d(M=(40, 30), n_steps=6, n_beams=2)
+    dp = DeviceProblem(p, solve_inputs(p, mat, det), ckpt_every=3)
+    ms, by = dp.rev_bench(3)
+    print(f"rev bench: {ms * 1e3:.1f} us, {by / 1e6:.2f} MB", flush=True)
+    assert ms > 0 and by > 0
+    # the problem still solves afterwards
+    I, rb, gen = dp.forward_adjoint(np.ones((p.n_beams, len(p.x_rays))))
+    assert np.all(np.isfinite(I))
+    dp.close()
