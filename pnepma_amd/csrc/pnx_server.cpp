@@ -679,8 +679,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
     }
     PhaseWork& w = plan.sub[e.sub].groups[e.key][e.phase];
     std::memcpy(h + e.n_off, &e.n_items, sizeof(int));
-    std::memcpy(h + e.half_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
-    std::memcpy(h + e.rev_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
+    if (!w.half.empty()) std::memcpy(h + e.half_off, w.half.data(), w.half.size() * sizeof(HalfStepArgs));
+    if (!w.rev.empty()) std::memcpy(h + e.rev_off, w.rev.data(), w.rev.size() * sizeof(RevArgs));
     std::memcpy(h + e.items_off, w.items.data(), w.items.size() * sizeof(int2));
   }
   if (hipMemcpyAsync(ln.dev_args, h, off, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
