@@ -355,6 +355,60 @@ def main_raster(args):
     print(json.dumps(out), flush=True)
 
 
+def main_raster3d(args):
+    """The 3D workload (P5 128x128x96, Al/Si inclusion) at its full shape for one
+    beam, on a sample of its energy grid: S steps of the CFL-0.5 step length from
+    the top of the energy range, forward+adjoint with checkpoints every 8.  The
+    whole solve (all CFL-0.5 steps, 64 beams) is extrapolated from the per-step
+    time, since one beam alone is minutes of GPU time."""
+    import numpy as np
+    import torch
+    from pnepma_amd import workloads, serving
+    from pnepma_amd.problem import solve_inputs
+    from pnepma_amd.solver import DeviceProblem
+    assert torch.cuda.is_available()
+    torch.cuda.init()
+    S = 16
+    full, mat, stds, det = workloads.inclusion3d(n_steps=16, n_beams=1)
+    n_cfl = serving.cfl_steps(full.N, full.gspec, mat, full.elements, full.E_initial, full.E_cutoff)
+    E1 = full.E_initial - (full.E_initial - full.E_cutoff) * S / n_cfl
+    p, mat, stds, det = workloads.inclusion3d(n_steps=S, n_beams=1, E0=full.E_initial, E1=E1)
+    si = solve_inputs(p, mat, det)
+    progress(f"3D problem prepared: {p.gspec.M}, {S} of {n_cfl} steps")
+    dp = DeviceProblem(p, si, ckpt_every=8)
+    ib = np.ones((1, len(p.x_rays)))
+    dp.forward_adjoint(ib)
+    torch.cuda.synchronize()
+    ms_fa = min(dp.forward_adjoint_timed(1) for _ in range(max(1, args.steps)))
+    ms_f = dp.forward_timed(2, True)
+    dp.close()
+    # state bytes per step: forward (even half step, fused odd pair), recompute, reverse
+    per_step = request_cost(p) / S
+    from pnepma_amd.grid import class_moments, odd_in
+    cm, M = class_moments(p.N), [a.M for a in p.gspec.axes]
+    vol = {0: 0.0, 1: 0.0}
+    for c in p.gspec.active_classes():
+        pts = 1
+        for a in range(3):
+            pts *= M[a] - 1 if odd_in(c, a) else M[a]
+        vol[bin(c).count("1") % 2] += pts * len(cm[c]) * 8.0
+    fwd_part = 3 * (vol[0] + vol[1])
+    bytes_step = per_step + fwd_part  # checkpointed: the forward runs twice
+    out = {
+        "metric": METRIC, "value": 1e3 * S / ms_fa, "unit": "energy steps/s (forward+adjoint, one beam)",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms_fa, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D P5 128x128x96, Al/Si with an ellipsoidal inclusion, one beam, "
+                               f"{S} of the {n_cfl} CFL-0.5 energy steps of 25 -> 1 keV, checkpoints every 8"},
+        "ms_per_energy_step_fwd_adj": ms_fa / S, "ms_per_energy_step_fwd": ms_f / S,
+        "state_bytes_per_energy_step_fwd_adj": bytes_step,
+        "achieved_state_GBps": bytes_step / (ms_fa / S * 1e-3) / 1e9,
+        "extrapolated_s_per_beam_fwd_adj": ms_fa / S * n_cfl / 1e3,
+        "extrapolated_s_64_beams_fwd_adj": 64 * ms_fa / S * n_cfl / 1e3,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main_kernel(args):
     """Only the dominant kernel (and a plain 256 MiB device copy for counter
     calibration): used under rocprofv3 to read per-dispatch HBM counters."""
@@ -379,7 +433,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "kernel"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "raster3d_step", "kernel"])
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tape-fraction", type=float, default=None,
@@ -393,6 +447,8 @@ def main():
         main_serve(args)
     elif args.workload == "kernel":
         main_kernel(args)
+    elif args.workload == "raster3d_step":
+        main_raster3d(args)
     else:
         main_raster(args)
 

@@ -90,3 +90,31 @@ def raster2d(N=7, M=(128, 96), n_steps=200, extent=(3 * UM, 2 * UM), E0=20e3, E1
     mat = MassFractionMaterial(els, CellFractions(vals, *edges, embedding=np.full(len(els), 1.0 / len(els))))
     stds = [MassFractionMaterial(els, ConstFractions(np.eye(len(els))[i])) for i in range(len(els))]
     return p, mat, stds, detector_40deg()
+
+
+def inclusion3d(N=5, M=(128, 128, 96), n_steps=300, extent=(2 * UM, 2 * UM, 1.5 * UM), E0=25e3, E1=1e3,
+                n_beams=64, seed=4, source_mode="lateral"):
+    """3D P5 Al/Si block with a seeded ellipsoidal Si-rich inclusion, scanned by an
+    8 x 8 beam raster in (y, z): the 3D forward+adjoint workload."""
+    rng = np.random.default_rng(seed)
+    names = ["Al", "Si"]
+    els = [element(s) for s in names]
+    g = GridSpec.make(M, (-extent[0], 0.0), (-extent[1] / 2, extent[1] / 2), (-extent[2] / 2, extent[2] / 2))
+    ys = np.linspace(-0.4 * extent[1], 0.4 * extent[1], 8)
+    zs = np.linspace(-0.4 * extent[2], 0.4 * extent[2], 8)
+    pos = [(0.0, float(y), float(z)) for y in ys for z in zs][:n_beams]
+    p = ForwardPNProblem(N=N, gspec=g, elements=els, beam_energy=(E0, SIGMA_E), beam_positions=pos,
+                         beam_sigma=50 * NM, beam_direction=((-1.0, 0.0, 0.0), KAPPA), E_initial=E0, E_cutoff=E1,
+                         n_steps=n_steps, source_mode=source_mode)
+    p.x_rays = [XRay(s) for s in names]
+    centre = [rng.uniform(-0.7, -0.3) * extent[0], rng.uniform(-0.2, 0.2) * extent[1],
+              rng.uniform(-0.2, 0.2) * extent[2]]
+    semi = [rng.uniform(0.15, 0.3) * e for e in extent]
+    X, Y, Z = np.meshgrid(*[a.points(0) for a in g.axes], indexing="ij")
+    inside = ((X - centre[0]) / semi[0]) ** 2 + ((Y - centre[1]) / semi[1]) ** 2 + ((Z - centre[2]) / semi[2]) ** 2 < 1
+    vals = np.empty(M + (2,))
+    vals[...] = (0.9, 0.1)
+    vals[inside] = (0.2, 0.8)
+    mat = MassFractionMaterial(els, CellFractions(vals, *[cell_edges(a) for a in g.axes], embedding=vals[0, 0, 0]))
+    stds = [MassFractionMaterial(els, ConstFractions(np.eye(2)[i])) for i in range(2)]
+    return p, mat, stds, detector_40deg()
