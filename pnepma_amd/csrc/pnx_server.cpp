@@ -7,7 +7,7 @@
 // requests tick by tick, independently of the other lanes, so that the
 // kernels of different lanes overlap on the device:
 //   * 1D lane: one march launch per tick for all PN orders; a workgroup runs a
-//     slice of up to kMarchOps operations of one request (march_next), with
+//     slice of operations of one request (march_next; more for lower orders), with
 //     workgroup barriers between the half steps;
 //   * 2D/3D lanes (split by PN order): one operation per request per tick; for
 //     every (PN order, dimension) group and every launch phase one kernel runs
@@ -59,7 +59,13 @@ constexpr size_t kAlign = 256;
 //   phase B:    last odd half step of a run | adjoint even half step.
 // (The pair and phase B are independent of each other.)
 enum Phase : int { kPhaseFirstOdd, kPhaseA, kPhasePair, kPhaseB, kPhases };
-constexpr int kMarchOps = 48;  // operations per 1D request per march launch
+constexpr int kMarchOps = 48;  // operations per P9 1D request per march launch
+// Operations per march launch by PN order: one launch serves every order and
+// lasts as long as its slowest request, and an operation's time grows with the
+// moment count (1D: 6, 12, 20, 30 for P3-P9), so lower orders take more steps.
+int march_ops_per_launch(int N) {
+  return N <= 3 ? 5 * kMarchOps : (N == 5 ? 5 * kMarchOps / 2 : (N == 7 ? 3 * kMarchOps / 2 : kMarchOps));
+}
 constexpr int kGridTicks = 4;  // operations per 2D/3D request per grid-lane launch batch
 // Active requests that keep a lane's launches full: beyond these a request
 // that cannot be taped waits for memory rather than run checkpointed.
@@ -602,7 +608,7 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
 void plan_march(pnx_server* s, Request& r, TickPlan& plan) {
   MarchArgs m = make_march(s, r);
   m.pos = r.pos;
-  m.n_ops = std::min(kMarchOps, r.ops_left);
+  m.n_ops = std::min(march_ops_per_launch(r.d.N), r.ops_left);
   for (int i = 0; i < m.n_ops; ++i) (void)march_next(r.pos, r.d.n_steps, r.g.K, r.g.nck);
   r.ops_left -= m.n_ops;
   plan.march.push_back(m);
