@@ -180,14 +180,31 @@ __global__ void beam_sum_kernel(const double* __restrict__ in, long stride, int 
 }
 
 // One block per beam (the body is shared with the march kernel).
-__global__ __launch_bounds__(kBlock) void reduce_intensity_kernel(const double* __restrict__ partial, long step_stride,
-                                                                  long beam_stride, int n_parts,
-                                                                  const double* __restrict__ rec, int rec_len,
-                                                                  int sigma_off, int n_steps, int n_lines, double dE,
-                                                                  double dvol, double* __restrict__ I_out) {
-  const int b = blockIdx.x;
-  reduce_intensity_body(partial + b * beam_stride, step_stride, n_parts, rec, rec_len, sigma_off, n_steps, n_lines, dE,
-                        dvol, I_out + b * n_lines);
+// Intensity reduction in two stages so that one request spreads over many CUs:
+// block (g, b) sums the parts [g chunk, (g + 1) chunk) of beam b over every
+// step into slot g of step 0 of the beam's partial buffer (free once the
+// forward is done: the forward writes steps 1..n), then one block per beam sums
+// the slots in order.  Deterministic.
+__global__ __launch_bounds__(kBlock) void reduce_intensity_parts_kernel(double* __restrict__ partial, long step_stride,
+                                                                        long beam_stride, int n_parts, int chunk,
+                                                                        const double* __restrict__ rec, int rec_len,
+                                                                        int sigma_off, int n_steps, int n_lines,
+                                                                        double dE, double dvol) {
+  const int g = blockIdx.x, b = blockIdx.y;
+  double* pb = partial + b * beam_stride;
+  const int t0 = g * chunk, t1 = t0 + chunk < n_parts ? t0 + chunk : n_parts;
+  reduce_intensity_body(pb, step_stride, t0, t1, rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol,
+                        pb + (long)g * n_lines);
+}
+
+__global__ void reduce_intensity_final_kernel(const double* __restrict__ partial, long beam_stride, int n_chunks,
+                                              int n_lines, double* __restrict__ I_out) {
+  const int b = blockIdx.x, r = threadIdx.x;
+  if (r >= n_lines) return;
+  gcd __restrict__ pb = glb(partial) + b * beam_stride;
+  double s = 0.0;
+  for (int g = 0; g < n_chunks; ++g) s += pb[(long)g * n_lines + r];
+  glb(I_out)[(long)b * n_lines + r] = s;
 }
 
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, long n) {
@@ -219,11 +236,16 @@ hipError_t launch_beam_sum(const double* in, long stride, int n_beams, long n, d
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_parts,
+hipError_t launch_reduce_intensity(double* partial, long step_stride, long beam_stride, int n_parts,
                                    const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
                                    double dE, double dvol, int n_beams, double* I_out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_intensity_kernel, dim3(n_beams), dim3(kBlock), 0, s, partial, step_stride, beam_stride,
-                     n_parts, fwd_rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol, I_out);
+  if (n_parts < 1 || n_lines < 1 || n_lines > kMaxLines) return hipErrorInvalidValue;
+  const int chunk = (n_parts + 31) / 32;  // at most 32 blocks per beam, each over chunk parts
+  const int n_chunks = (n_parts + chunk - 1) / chunk;
+  hipLaunchKernelGGL(reduce_intensity_parts_kernel, dim3(n_chunks, n_beams), dim3(kBlock), 0, s, partial, step_stride,
+                     beam_stride, n_parts, chunk, fwd_rec, rec_len, sigma_off, n_steps, n_lines, dE, dvol);
+  hipLaunchKernelGGL(reduce_intensity_final_kernel, dim3(n_beams), dim3(64), 0, s, partial, beam_stride, n_chunks,
+                     n_lines, I_out);
   return hipGetLastError();
 }
 

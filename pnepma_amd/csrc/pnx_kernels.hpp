@@ -199,8 +199,9 @@ hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
 hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);
 // I[b][r] = sum_k dE*dvol*sigma_r(rec_k)*w_k * sum_t partial[b][k][t][r], k = 1..n_steps,
-// t over the n_parts = n_tiles * kWavesPerBlock partial sums of a step
-hipError_t launch_reduce_intensity(const double* partial, long step_stride, long beam_stride, int n_parts,
+// t over the n_parts = n_tiles * kWavesPerBlock partial sums of a step.  Step 0
+// of each beam's partial buffer is overwritten (scratch of the first stage).
+hipError_t launch_reduce_intensity(double* partial, long step_stride, long beam_stride, int n_parts,
                                    const double* fwd_rec, int rec_len, int sigma_off, int n_steps, int n_lines,
                                    double dE, double dvol, int n_beams, double* I_out, hipStream_t s);
 // out[i] = sum_b in[b*stride + i], i < n

@@ -369,45 +369,37 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Intensities of one beam from the per-step partial sums (n_parts per step)
-// (integrate_step_and_add!, x_ray_intensities.jl:172-191).  Every thread of
-// the block must call it.
-__device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int n_parts,
-                                                      const double* __restrict__ rec, int rec_len, int sigma_off,
-                                                      int n_steps, int n_lines, double dE, double dvol,
-                                                      double* __restrict__ I_out) {
-  // I_r = dE dvol sum_k sigma_k,r sum_t partial_k,t,r.  The weight depends only
-  // on the step and the line, so the two sums are exchanged: a thread takes
-  // parts t and runs over the steps with independent loads, and one block
-  // reduction over the parts finishes (summing each step's parts first made
-  // every thread a chain of hundreds of dependent loads).
+// Intensity sums of one beam over the parts [t0, t1) of the per-step partial
+// sums (integrate_step_and_add!, x_ray_intensities.jl:172-191): thread t sums
+// the steps k = 1 + t, 1 + t + 256, ..., each over the parts, then the block
+// reduces and writes out[r].  Every thread of the block must call it.
+__device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int t0,
+                                                      int t1, const double* __restrict__ rec, int rec_len,
+                                                      int sigma_off, int n_steps, int n_lines, double dE, double dvol,
+                                                      double* __restrict__ out) {
   __shared__ double red[kBlock / 64][kMaxLines];
-  const int nl = n_lines < kMaxLines ? n_lines : kMaxLines;
   double acc[kMaxLines];
 #pragma unroll
   for (int r = 0; r < kMaxLines; ++r) acc[r] = 0.0;
-  gcd __restrict__ sg = glb(rec) + sigma_off;
-  for (int t = threadIdx.x; t < n_parts; t += kBlock) {
-    gcd __restrict__ pt = glb(partial) + (long)t * n_lines;
-#pragma unroll 4
-    for (int kk = 1; kk <= n_steps; ++kk) {
-      gcd __restrict__ ps = pt + kk * step_stride;
-      gcd __restrict__ sig = sg + (long)kk * rec_len;
-#pragma unroll
-      for (int r = 0; r < kMaxLines; ++r)
-        if (r < nl) acc[r] += sig[r] * ps[r];
+  for (int kk = 1 + threadIdx.x; kk <= n_steps; kk += kBlock) {
+    gcd __restrict__ ps = glb(partial) + kk * step_stride;
+    gcd __restrict__ sig = glb(rec) + (long)kk * rec_len + sigma_off;
+    for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
+      double integral = 0.0;
+      for (int t = t0; t < t1; ++t) integral += ps[(long)t * n_lines + r];
+      acc[r] += dE * dvol * sig[r] * integral * 1.0;
     }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r = 0; r < nl; ++r) {
+  for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
     const double s = wave_sum(acc[r]);
     if (lane == 0) red[wave][r] = s;
   }
   __syncthreads();
-  if (threadIdx.x < nl) {
+  if (threadIdx.x < n_lines && threadIdx.x < kMaxLines) {
     double s = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-    glb(I_out)[threadIdx.x] = dE * dvol * s;
+    glb(out)[threadIdx.x] = s;
   }
   __syncthreads();
 }

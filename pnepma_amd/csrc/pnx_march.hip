@@ -80,7 +80,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
         half_pass<N, 1, kForward>(m.fwd, StepView{dst, dst, dst, rec, nullptr, h});
       }
       if (o.reduce)
-        reduce_intensity_body(m.partial, m.partial_step, L.n_tiles * kWavesPerBlock, m.rec_f, m.rec_len, m.sigma_off, m.n, L.n_lines,
+        reduce_intensity_body(m.partial, m.partial_step, 0, L.n_tiles * kWavesPerBlock, m.rec_f, m.rec_len, m.sigma_off, m.n, L.n_lines,
                               dE, L.dvol, m.I_out);
     } else if (o.kind == kOpReverse) {
       const double* rec = m.rec_a + (long)o.step * m.rec_len;
