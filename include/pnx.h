@@ -183,11 +183,13 @@ int pnx_rev_bench(pnx_problem* p, int reps, double* ms_out, double* bytes_out);
  * A continuous-batching server of forward+adjoint requests with mixed PN
  * order, grid and energy range.  Requests are admitted, in arrival order, as
  * soon as their arrival time has passed and their device memory fits in the
- * HBM budget; every scheduler tick advances each admitted request by one
- * energy step (forward, recomputation or reverse) in persistent kernels that
- * batch all requests of one PN order and dimension.  The reference has no
- * counterpart (it solves one problem at a time); each request runs the same
- * march as pnx_forward_adjoint with one beam. */
+ * HBM budget (taped or checkpointed, see pnx_server_set_tape_limit).  Every
+ * scheduler tick advances each admitted 2D/3D request by one operation of its
+ * sequence (a forward, recomputed or reverse energy step) in grouped kernels
+ * that batch all requests of one PN order and dimension, and each 1D request
+ * by a slice of operations in one march kernel.  The reference has no
+ * counterpart (it solves one problem at a time); each request runs the march
+ * of pnx_forward_adjoint with one beam. */
 typedef struct pnx_server pnx_server;
 
 typedef struct {
