@@ -243,8 +243,8 @@ int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, in
 /* Device bytes a request would hold while admitted. */
 int64_t pnx_request_device_bytes(const pnx_desc* desc);
 /* The operation sequence the server runs for a request with n energy steps and
- * checkpoints every K (K = 0: taped, 2n + 1 operations), for inspection: for
- * operation t < 3n + 1,
+ * checkpoints every K (K = 0: taped, 2n + 1 operations; else 3n + 1), for
+ * inspection: for each operation t,
  * out[9t .. 9t+8] = kind (1 forward, 2 recompute, 3 reverse, 4 final),
  * energy step, checkpoint slot written before the step (-1: none), source
  * buffer and slot, destination buffer and slot (0 state, 1 checkpoints,
