@@ -193,13 +193,14 @@ def cpu_sample(specs, preps, budget_s=24.0, candidates=256):
     the first one taken is the first of at most the stratum's median cost.
     The whole stream's oracle time is then predicted from each request's cost
     and its stratum's measured seconds per cost unit.  Returns the CPU
-    throughput in solves/s, the sample size and its seconds."""
+    throughput in solves/s, the sample size, its seconds and the sampled
+    request indices."""
     import numpy as np
     from oracle import oracle
     cost = np.array([request_cost(s.problem) for s in specs])
     multi = np.array([s.problem.gspec.axes[1].M > 1 for s in specs])
     head = np.arange(min(candidates, len(specs)))
-    sec_per_cost, n_all, el_all = {}, 0, 0.0
+    sec_per_cost, n_all, el_all, taken = {}, 0, 0.0, []
     for stratum, share in ((False, 0.2), (True, 0.8)):
         idx = head[multi[head] == stratum]
         if idx.size == 0:
@@ -216,15 +217,44 @@ def cpu_sample(specs, preps, budget_s=24.0, candidates=256):
             el += time.perf_counter() - t
             done += cost[i]
             n += 1
+            taken.append(int(i))
             if el > 0.75 * b:
                 break
         sec_per_cost[stratum] = el / done
         n_all += n
         el_all += el
     if not sec_per_cost:
-        return 0.0, 0, 0.0
+        return 0.0, 0, 0.0, taken
     spc = np.array([sec_per_cost.get(bool(m), next(iter(sec_per_cost.values()))) for m in multi])
-    return len(specs) / float(np.sum(cost * spc)), n_all, el_all
+    return len(specs) / float(np.sum(cost * spc)), n_all, el_all, taken
+
+
+_CPU_SAMPLE = None
+
+
+def _run_sample(_):
+    specs, preps, idx = _CPU_SAMPLE
+    from oracle import oracle
+    t = time.perf_counter()
+    for i in idx:
+        oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
+    return time.perf_counter() - t
+
+
+def cpu_all_cores(specs, preps, idx, single_s, workers):
+    """The same oracle sample run as `workers` concurrent copies, one process
+    each (before the GPU is touched).  Every copy takes longer than the one-core
+    run when the cores contend for memory; the all-core throughput is the
+    one-core throughput times workers over that slowdown.  Returns (slowdown,
+    wall seconds)."""
+    import multiprocessing as mp
+    global _CPU_SAMPLE
+    _CPU_SAMPLE = (specs, preps, idx)
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        times = pool.map(_run_sample, range(workers), chunksize=1)
+    wall = time.perf_counter() - t
+    return sum(times) / len(times) / single_s, wall
 
 
 def progress(msg):
@@ -235,6 +265,13 @@ def main_serve(args):
     import numpy as np
     specs, preps = prepare_stream(args.requests)
     progress(f"prepared {len(preps)} requests")
+    # CPU baselines first, while no process has touched the GPU
+    cpu_rate, n_cpu, el_cpu, cpu_idx = cpu_sample(specs, preps) if not args.no_cpu else (0.0, 0, 0.0, [])
+    progress(f"cpu sample: {n_cpu} requests in {el_cpu:.1f} s")
+    workers = max(1, min(16, os.cpu_count() or 1))
+    if n_cpu:
+        slowdown, wall_all = cpu_all_cores(specs, preps, cpu_idx, el_cpu, workers)
+        progress(f"cpu sample x{workers} processes: {wall_all:.1f} s, slowdown per copy {slowdown:.2f}")
     import torch
     assert torch.cuda.is_available()
     torch.cuda.init()
@@ -278,8 +315,6 @@ def main_serve(args):
     progress(f"plain loop: {len(seq_t)} requests, {seq_rate:.2f} solves/s")
     # the plain-loop sample is the head of the stream: compare with the served results
     alone_err = max(float(np.max(np.abs(a - pr.I_out)) / np.max(np.abs(a))) for a, pr in zip(seq_I, preps))
-    cpu_rate, n_cpu, el_cpu = cpu_sample(specs, preps) if not args.no_cpu else (0.0, 0, 0.0)
-    progress(f"cpu sample: {n_cpu} requests in {el_cpu:.1f} s")
     roof = roofline_probe()
     out = {
         "metric": METRIC, "value": served, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -313,6 +348,12 @@ def main_serve(args):
                                          "is predicted from each request's algorithmic bytes (bench.request_cost) "
                                          "and its dimension's measured seconds per byte"}
         out["speedup_vs_cpu"] = served / cpu_rate
+        all_rate = cpu_rate * workers / slowdown
+        out["cpu_baseline_all_cores"] = {"value": all_rate, "unit": "solves/s", "cores": workers, "kind": "port",
+                                         "sample": f"the same sample as {workers} concurrent processes "
+                                                   f"({wall_all:.1f} s wall; each copy {slowdown:.2f}x its "
+                                                   "one-core time); rate = one-core rate x cores / slowdown"}
+        out["speedup_vs_cpu_all_cores"] = served / all_rate
     print(json.dumps(out), flush=True)
 
 
