@@ -25,6 +25,8 @@ struct Layout {
   int n_cells;      // Mx*My*Mz unit cells
   int tiles_x, tiles_y;  // tiles per row and per column of one z-plane
   int n_tiles;      // tiles_x * tiles_y * Mz
+  int cw_tiles_x;   // 2D passes with one class per wavefront: cw tiles per row
+  int n_cw_items;   // and blocks per beam (4 per cw tile; 0 unless dims == 2)
   int rec_len;
 };
 
@@ -36,6 +38,13 @@ constexpr int kBlock = 256;
 // the other parity twice than a run of two whole rows, and a wavefront still
 // covers whole 128-byte lines (two row segments of 32 doubles).
 constexpr int tile_x(int dims) { return dims == 1 ? kBlock : 32; }
+
+// The 2D passes with one class per wavefront (the fused reverse pass) tile
+// the plane with their own kCwTileX x kCwTileX tiles, each taken by four
+// blocks of 64 cells (kCwTileX / 4 rows).  Against 32 x 8 tiles of two-row
+// blocks this took the fused reverse pass at 2D P7 from 937 to 915 us, while
+// the half steps measured faster with 32 x 8.
+constexpr int kCwTileX = 16;
 
 // The forward even half step leaves one intensity partial sum per wavefront
 // (no block-wide reduction, no barrier): kBlock / 64 per tile, per line.
@@ -146,8 +155,8 @@ hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const
                                     const int2* items, const int* n_items, int grid, hipStream_t s);
 // Phase launches of the server's grid lanes: one block per item of a device
 // work list whose entries carry their kind (item.x = kind << kItemKindShift |
-// argument index, item.y = tile; a 2D reverse item's tile is tile * 4 + the
-// two-row sub-tile of its block).
+// argument index, item.y = tile; a 2D reverse item's is cw tile * 4 + the
+// 64-cell sub-tile of its block).
 //   phase A: kItemFwdEven (half[arg], forward even half step) and kItemRev (rev[arg]);
 //   pair:    kItemPair (half[arg], fused forward odd pair);
 //   phase B: kItemFwdOdd (forward odd half step) and kItemAdjEven (adjoint even half step).

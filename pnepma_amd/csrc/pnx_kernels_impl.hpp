@@ -712,18 +712,22 @@ __device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
   });
 }
 
-// 2D form with one class per wavefront: a block takes two rows (sub-tile
-// `sub` of 4) of a 32 x 8 tile, and wavefront w the class 2w (eee, eoe, oee,
-// ooe) at those 64 cells.  Each wavefront runs one class's code, so the
-// kernel needs the registers of one class instead of all four, which raises
-// the occupancy of these register-heavy passes; a wavefront still reads two
-// row segments of 32 doubles per moment.
+// 2D form with one class per wavefront: block `item` takes 64 cells, the
+// rows of sub-tile item & 3 of cw tile item >> 2 (kCwTileX x kCwTileX cells,
+// tiles x-fastest), and wavefront w the class 2w (eee, eoe, oee, ooe) at
+// those cells.  Each wavefront runs one class's code, so the kernel needs
+// the registers of one class instead of all four, which raises the occupancy
+// of these register-heavy passes; a wavefront reads whole row segments of
+// kCwTileX doubles per moment.
 template <int N, class F>
-__device__ __forceinline__ void class_per_wave_2d(const Layout& L, int tile, int sub, F&& f) {
+__device__ __forceinline__ void class_per_wave_2d(const Layout& L, int item, F&& f) {
+  constexpr int SUB_ROWS = 64 / kCwTileX;
+  static_assert(64 % kCwTileX == 0 && kCwTileX == 4 * SUB_ROWS, "a sub-tile is 64 cells, a cw tile four");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tx = tile % L.tiles_x, ty = (tile / L.tiles_x) % L.tiles_y;
-  const int i = tx * tile_x(2) + (lane & 31);
-  const int j = ty * (kBlock / tile_x(2)) + sub * 2 + (lane >> 5);
+  const int tile = item >> 2, sub = item & 3;
+  const int tx = tile % L.cw_tiles_x, ty = tile / L.cw_tiles_x;
+  const int i = tx * kCwTileX + lane % kCwTileX;
+  const int j = ty * kCwTileX + sub * SUB_ROWS + lane / kCwTileX;
   if (i >= L.Mx || j >= L.My) return;
   const long p = i + (long)L.X * j;
   static_for<0, 4>([&](auto Wc) {
@@ -738,12 +742,12 @@ __device__ __forceinline__ void class_per_wave_2d(const Layout& L, int tile, int
 }
 
 template <int N>
-__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int tile, int sub) {
+__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int item) {
   const Layout& L = a.adj.L;
   const long bo = (long)beam * L.beam_stride;
   gcd lam_b = glb(a.adj.from) + bo;
   gcd u_b = glb(a.dens.u) + bo;
-  class_per_wave_2d<N>(L, tile, sub, [&](auto Cc, int i, int j, long p) {
+  class_per_wave_2d<N>(L, item, [&](auto Cc, int i, int j, long p) {
     constexpr int C = decltype(Cc)::value;
     if constexpr (parity_of(C) == 1)
       rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
@@ -758,9 +762,9 @@ constexpr int rev_cw_min_blocks(int N) { return N <= 7 ? 3 : 1; }
 
 template <int N>
 __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(RevArgs a) {
-  int beam, t4;
-  block_coords(a.adj.L.n_tiles * 4, a.adj.n_beams, beam, t4);
-  rev_body_cw<N>(a, beam, t4 >> 2, t4 & 3);
+  int beam, item;
+  block_coords(a.adj.L.n_cw_items, a.adj.n_beams, beam, item);
+  rev_body_cw<N>(a, beam, item);
 }
 
 // Phase kernels of the grid lanes: the kinds of one phase share a launch, so
@@ -776,7 +780,7 @@ __global__ __launch_bounds__(kBlock, DIM == 2 ? rev_cw_min_blocks(N) : 1) void p
   const int kind = w.x >> kItemKindShift, arg = w.x & ((1 << kItemKindShift) - 1);
   if (kind == kItemRev) {
     if constexpr (DIM == 2)
-      rev_body_cw<N>(rev[arg], 0, w.y >> 2, w.y & 3);
+      rev_body_cw<N>(rev[arg], 0, w.y);
     else
       rev_body<N, DIM>(rev[arg], 0, w.y);
   } else {
@@ -1099,7 +1103,7 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
   const dim3 grid(a.adj.L.n_tiles * a.adj.n_beams), block(kBlock);
   switch (dims) {
     case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(grid.x * 4), block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a); break;
     case 3: hipLaunchKernelGGL((rev_kernel<N, 3>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }

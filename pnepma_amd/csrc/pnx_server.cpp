@@ -539,9 +539,9 @@ void add_half(PhaseWork& w, const HalfStepArgs& a, int kind) {
 void add_rev(PhaseWork& w, const RevArgs& a) {
   const int idx = (int)w.rev.size();
   w.rev.push_back(a);
-  // 2D: one class per wavefront, one block per two-row sub-tile
-  const int per_tile = a.adj.L.dims == 2 ? 4 : 1;
-  for (int t = 0; t < a.adj.L.n_tiles * per_tile; ++t) w.items.push_back(make_int2(kItemRev << kItemKindShift | idx, t));
+  // 2D: one class per wavefront, one block per 64-cell sub-tile of a cw tile
+  const int n = a.adj.L.dims == 2 ? a.adj.L.n_cw_items : a.adj.L.n_tiles;
+  for (int t = 0; t < n; ++t) w.items.push_back(make_int2(kItemRev << kItemKindShift | idx, t));
 }
 
 // Plan the next operation of request r on a grid lane.
