@@ -89,7 +89,7 @@ int make_geometry(const pnx_desc& d, Geometry& g, std::string& err) {
     L.tiles_y = (L.My + ty - 1) / ty;
     L.n_tiles = L.tiles_x * L.tiles_y * L.Mz;
     L.cw_tiles_x = (L.Mx + kCwTileX - 1) / kCwTileX;
-    L.n_cw_items = dims == 2 ? 4 * L.cw_tiles_x * ((L.My + kCwTileX - 1) / kCwTileX) : 0;
+    L.n_cw_items = dims >= 2 ? 4 * L.cw_tiles_x * ((L.My + kCwTileX - 1) / kCwTileX) * L.Mz : 0;
   }
   g.rec_len = d.n_elements + d.n_elements * (d.N + 1) + 1 + d.n_lines;
   L.rec_len = g.rec_len;

@@ -767,6 +767,47 @@ __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(Re
   rev_body_cw<N>(a, beam, item);
 }
 
+// 3D batch form of the fused reverse pass with one class per wavefront: two
+// blocks take the same 64 cells of one z-plane (sub-tile item & 3 of cw tile
+// item >> 2, cw tiles x-fastest, then planes), the first the classes 0-3 and
+// the second 4-7, wavefront w of block half h the class 4h + w; 2 n_cw_items
+// blocks per beam.  One thread per cell over all eight classes needed 179
+// VGPRs at P5 (occupancy 2).
+template <int N>
+__global__ __launch_bounds__(kBlock) void rev_kernel_cw3(RevArgs a) {
+  const Layout& L = a.adj.L;
+  int beam, item2;
+  block_coords(2 * a.adj.L.n_cw_items, a.adj.n_beams, beam, item2);
+  const int half = item2 & 1, item = item2 >> 1;
+  constexpr int SUB_ROWS = 64 / kCwTileX;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cw_tiles_y = (L.My + kCwTileX - 1) / kCwTileX;
+  const int sub = item & 3, r = item >> 2;
+  const int tile = r % (L.cw_tiles_x * cw_tiles_y), k = r / (L.cw_tiles_x * cw_tiles_y);
+  const int tx = tile % L.cw_tiles_x, ty = tile / L.cw_tiles_x;
+  const int i = tx * kCwTileX + lane % kCwTileX;
+  const int j = ty * kCwTileX + sub * SUB_ROWS + lane / kCwTileX;
+  if (i >= L.Mx || j >= L.My || k >= L.Mz) return;
+  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const long bo = (long)beam * L.beam_stride;
+  gcd lam_b = glb(a.adj.from) + bo;
+  gcd u_b = glb(a.dens.u) + bo;
+  static_for<0, 8>([&](auto Cc) {
+    constexpr int C = decltype(Cc)::value;
+    if constexpr (ncls<N, C>() > 0) {
+      if (4 * half + wave == C) {
+        const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0) || (odd_in(C, 2) && k == 0));
+        if (interior) {
+          if constexpr (parity_of(C) == 1)
+            rev_class<N, 3, C>(a, lam_b, u_b, i, j, k, p, beam);
+          else
+            dens_class<N, 3, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
+        }
+      }
+    }
+  });
+}
+
 // Phase kernels of the grid lanes: the kinds of one phase share a launch, so
 // a sub-tick is two launches over all its requests instead of one per kind.
 // The registers are those of the phase's largest kind.
@@ -1104,7 +1145,9 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
   switch (dims) {
     case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((rev_kernel<N, 3>), grid, block, 0, s, a); break;
+    case 3:
+      hipLaunchKernelGGL((rev_kernel_cw3<N>), dim3(2 * a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -68,6 +68,27 @@ def test_step_forward_3d(oracle_lib):
     _step_case(p, mat, det, oracle_lib, adjoint=False, record=3)
 
 
+@pytest.mark.parametrize("ckpt", [0, 4])
+def test_forward_adjoint_3d(oracle_lib, ckpt):
+    """3D P5 forward + reference adjoint of one beam against the oracle: I and rho_bar of all eight classes."""
+    from pnepma_amd.grid import GridSpec
+    from pnepma_amd.material import ConstFractions, MassFractionMaterial
+    p, mat, _, det = small_2d(N=5, M=(8, 6), n_steps=10)
+    p.gspec = GridSpec.make((9, 7, 5), (-0.3e-4, 0.0), (-1e-5, 1e-5), (-0.8e-5, 0.8e-5))
+    p.beam_positions = [(0.0, 0.0, 0.0)]
+    mat = MassFractionMaterial(mat.elements, ConstFractions([0.3, 0.3, 0.4]))
+    si = inputs(p, mat, det)
+    dp = DeviceProblem(p, si, ckpt_every=ckpt)
+    ib = np.array([[1.0, -0.5, 0.25]])
+    I, rb, gen = dp.forward_adjoint(ib)
+    I_ref, rb_ref, _, _ = oracle_lib.forward_adjoint(p, si, ib[0], beam=0)
+    assert rel_err(I[0], I_ref) < 1e-10
+    for c in p.gspec.active_classes():
+        e = rel_err(rb[c], rb_ref[c])
+        print(f"rho_bar {G.CLASS_NAMES[c]} rel err {e:.3e}", flush=True)
+        assert e < 1e-8
+
+
 def test_forward_intensities_1d(oracle_lib):
     p, mat, stds, det = small_1d(N=5, cells=60, n_steps=60)
     si = inputs(p, mat, det)
