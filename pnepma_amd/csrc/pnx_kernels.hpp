@@ -156,7 +156,8 @@ hipError_t launch_half_step_grouped(int N, int dims, int parity, int mode, const
 // Phase launches of the server's grid lanes: one block per item of a device
 // work list whose entries carry their kind (item.x = kind << kItemKindShift |
 // argument index, item.y = tile; a 2D reverse item's is cw tile * 4 + the
-// 64-cell sub-tile of its block).
+// 64-cell sub-tile of its block, a 3D one's twice that plus the half of the
+// eight classes it updates).
 //   phase A: kItemFwdEven (half[arg], forward even half step) and kItemRev (rev[arg]);
 //   pair:    kItemPair (half[arg], fused forward odd pair);
 //   phase B: kItemFwdOdd (forward odd half step) and kItemAdjEven (adjoint even half step).

@@ -767,17 +767,16 @@ __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(Re
   rev_body_cw<N>(a, beam, item);
 }
 
-// 3D batch form of the fused reverse pass with one class per wavefront: two
-// blocks take the same 64 cells of one z-plane (sub-tile item & 3 of cw tile
+// 3D form of the fused reverse pass with one class per wavefront: two blocks
+// take the same 64 cells of one z-plane (sub-tile item & 3 of cw tile
 // item >> 2, cw tiles x-fastest, then planes), the first the classes 0-3 and
-// the second 4-7, wavefront w of block half h the class 4h + w; 2 n_cw_items
-// blocks per beam.  One thread per cell over all eight classes needed 179
-// VGPRs at P5 (occupancy 2).
+// the second 4-7, wavefront w of block half h the class 4h + w; block item2
+// is (item, half) = (item2 >> 1, item2 & 1), 2 n_cw_items blocks per beam.
+// One thread per cell over all eight classes needed 179 VGPRs at P5
+// (occupancy 2).
 template <int N>
-__global__ __launch_bounds__(kBlock) void rev_kernel_cw3(RevArgs a) {
+__device__ __forceinline__ void rev_body_cw3(const RevArgs& a, int beam, int item2) {
   const Layout& L = a.adj.L;
-  int beam, item2;
-  block_coords(2 * a.adj.L.n_cw_items, a.adj.n_beams, beam, item2);
   const int half = item2 & 1, item = item2 >> 1;
   constexpr int SUB_ROWS = 64 / kCwTileX;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -808,6 +807,13 @@ __global__ __launch_bounds__(kBlock) void rev_kernel_cw3(RevArgs a) {
   });
 }
 
+template <int N>
+__global__ __launch_bounds__(kBlock) void rev_kernel_cw3(RevArgs a) {
+  int beam, item2;
+  block_coords(2 * a.adj.L.n_cw_items, a.adj.n_beams, beam, item2);
+  rev_body_cw3<N>(a, beam, item2);
+}
+
 // Phase kernels of the grid lanes: the kinds of one phase share a launch, so
 // a sub-tick is two launches over all its requests instead of one per kind.
 // The registers are those of the phase's largest kind.
@@ -822,6 +828,8 @@ __global__ __launch_bounds__(kBlock, DIM == 2 ? rev_cw_min_blocks(N) : 1) void p
   if (kind == kItemRev) {
     if constexpr (DIM == 2)
       rev_body_cw<N>(rev[arg], 0, w.y);
+    else if constexpr (DIM == 3)
+      rev_body_cw3<N>(rev[arg], 0, w.y);
     else
       rev_body<N, DIM>(rev[arg], 0, w.y);
   } else {
