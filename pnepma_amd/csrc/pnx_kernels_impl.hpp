@@ -304,10 +304,12 @@ __device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restri
   });
 }
 
-// The own values of the updated class are read once and written once, by the
-// thread of their point only: nontemporal accesses keep these streams from
-// evicting the lines the stencils share (other-parity rows, densities).  On
-// the 2D P7 160-beam even half step this took 210 to 200 us per launch.
+// Streams that only the thread of their point reads and writes, once per
+// pass: the own values of the updated class in a half step, the odd-class
+// adjoint at its point, rho_bar and the generation sums in the reverse pass.
+// Nontemporal accesses keep them from evicting the lines the stencils share
+// (other-parity rows, densities).  On the 2D P7 160-beam even half step this
+// took 210 to 200 us per launch.
 __device__ __forceinline__ double stream_load(gcd q) { return __builtin_nontemporal_load(q); }
 __device__ __forceinline__ void stream_store(gd q, double v) { __builtin_nontemporal_store(v, q); }
 
@@ -558,7 +560,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
   double u[NC], du[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
-    u[m] = stream_load(uc + m * V + p);
+    u[m] = uc[m * V + p];  // shared: the other parity's stencils read it too
     du[m] = 0.0;
   }
   const double gfac = rec[ne + ne * (N + 1)];
@@ -579,7 +581,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
   for (int e0 = 0; e0 < ne; e0 += kBatch) {
     double acc[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) acc[b] = rb[batch_index(e0, b, ne) * V + p];
+    for (int b = 0; b < kBatch; ++b) acc[b] = stream_load(rb + batch_index(e0, b, ne) * V + p);
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
       const int e = e0 + b;
@@ -590,7 +592,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
           constexpr int l = Tab<N>::t.l[C][I];
           acc[b] += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
         });
-        rb[e * V + p] = acc[b];
+        stream_store(rb + e * V + p, acc[b]);
       }
     }
   }
@@ -602,13 +604,13 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
     for (int r0 = 0; r0 < nl; r0 += kBatch) {
       double gv[kBatch];
 #pragma unroll
-      for (int b = 0; b < kBatch; ++b) gv[b] = g[batch_index(r0, b, nl) * V + p];
+      for (int b = 0; b < kBatch; ++b) gv[b] = stream_load(g + batch_index(r0, b, nl) * V + p);
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
         const int r = r0 + b;
         if (r < nl) {
           const double integral = a.dE * w * sig[r] * int_fac * ib[r];
-          g[r * V + p] = gv[b] + u[0] * integral;
+          stream_store(g + r * V + p, gv[b] + u[0] * integral);
         }
       }
     }
@@ -649,7 +651,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
   gd out = glb(a.adj.to_dst) + (long)beam * L.beam_stride + L.cls_off[C];
   double l0[NC], l1[NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) l0[m] = lc[m * V + p];
+  for (int m = 0; m < NC; ++m) l0[m] = stream_load(lc + m * V + p);
   if constexpr (N < 9) {
     if (a.rec3 != nullptr) {
       // The pullback reads lambda after the last odd half step of step i + 1
@@ -674,7 +676,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
         for (int m = 0; m < NC; ++m) l1[m] = l0[m];
       }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) out[m * V + p] = l1[m];
+      for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, l1[m]);
       dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
       return;
     }
@@ -692,7 +694,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
     for (int m = 0; m < NC; ++m) l1[m] = l0[m];
   }
 #pragma unroll
-  for (int m = 0; m < NC; ++m) out[m * V + p] = l1[m];
+  for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, l1[m]);
 }
 
 template <int N, int DIM>
