@@ -309,9 +309,19 @@ __device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restri
 // adjoint at its point, rho_bar and the generation sums in the reverse pass.
 // Nontemporal accesses keep them from evicting the lines the stencils share
 // (other-parity rows, densities).  On the 2D P7 160-beam even half step this
-// took 210 to 200 us per launch.
-__device__ __forceinline__ double stream_load(gcd q) { return __builtin_nontemporal_load(q); }
-__device__ __forceinline__ void stream_store(gd q, double v) { __builtin_nontemporal_store(v, q); }
+// took 210 to 200 us per launch.  In 1D the accesses stay ordinary: a 1D
+// request's state is a few hundred KB, and the march kernel reads what one
+// pass wrote in the next, after a barrier, from L2.
+template <int DIM>
+__device__ __forceinline__ double stream_load(gcd q) {
+  if constexpr (DIM >= 2) return __builtin_nontemporal_load(q);
+  else return *q;
+}
+template <int DIM>
+__device__ __forceinline__ void stream_store(gd q, double v) {
+  if constexpr (DIM >= 2) __builtin_nontemporal_store(v, q);
+  else *q = v;
+}
 
 // u0 receives the updated zeroth moment of eee (forward even half step), from
 // which the block forms the intensity contributions.
@@ -327,11 +337,11 @@ __device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restri
   gcd uc = src + L.cls_off[C];
   double u[NC], un[NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) u[m] = stream_load(uc + m * V + p);
+  for (int m = 0; m < NC; ++m) u[m] = stream_load<DIM>(uc + m * V + p);
   update_core<N, DIM, C, MODE>(a, rec, dt, from, u, un, i, j, k, p, beam);
   gd out = dst + L.cls_off[C];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, un[m]);
+  for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, un[m]);
   if constexpr (MODE == kForward && PAR == 0 && C == 0) *u0 = un[0];
 }
 
@@ -350,7 +360,7 @@ __device__ __forceinline__ void update_class2(const HalfStepArgs& a, const StepV
   double u[NC], u1[NC], u2[NC], du[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) {
-    u[m] = stream_load(uc + m * V + p);
+    u[m] = stream_load<DIM>(uc + m * V + p);
     du[m] = 0.0;
   }
   gcd __restrict__ rec = glb(v.rec);
@@ -365,11 +375,11 @@ __device__ __forceinline__ void update_class2(const HalfStepArgs& a, const StepV
   if (dst) {
     gd out = dst + L.cls_off[C];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, u1[m]);
+    for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, u1[m]);
   }
   gd out2 = dst2 + L.cls_off[C];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) stream_store(out2 + m * V + p, u2[m]);
+  for (int m = 0; m < NC; ++m) stream_store<DIM>(out2 + m * V + p, u2[m]);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -581,7 +591,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
   for (int e0 = 0; e0 < ne; e0 += kBatch) {
     double acc[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) acc[b] = stream_load(rb + batch_index(e0, b, ne) * V + p);
+    for (int b = 0; b < kBatch; ++b) acc[b] = stream_load<DIM>(rb + batch_index(e0, b, ne) * V + p);
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
       const int e = e0 + b;
@@ -592,7 +602,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
           constexpr int l = Tab<N>::t.l[C][I];
           acc[b] += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
         });
-        stream_store(rb + e * V + p, acc[b]);
+        stream_store<DIM>(rb + e * V + p, acc[b]);
       }
     }
   }
@@ -604,13 +614,13 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
     for (int r0 = 0; r0 < nl; r0 += kBatch) {
       double gv[kBatch];
 #pragma unroll
-      for (int b = 0; b < kBatch; ++b) gv[b] = stream_load(g + batch_index(r0, b, nl) * V + p);
+      for (int b = 0; b < kBatch; ++b) gv[b] = stream_load<DIM>(g + batch_index(r0, b, nl) * V + p);
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
         const int r = r0 + b;
         if (r < nl) {
           const double integral = a.dE * w * sig[r] * int_fac * ib[r];
-          stream_store(g + r * V + p, gv[b] + u[0] * integral);
+          stream_store<DIM>(g + r * V + p, gv[b] + u[0] * integral);
         }
       }
     }
@@ -651,7 +661,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
   gd out = glb(a.adj.to_dst) + (long)beam * L.beam_stride + L.cls_off[C];
   double l0[NC], l1[NC];
 #pragma unroll
-  for (int m = 0; m < NC; ++m) l0[m] = stream_load(lc + m * V + p);
+  for (int m = 0; m < NC; ++m) l0[m] = stream_load<DIM>(lc + m * V + p);
   if constexpr (N < 9) {
     if (a.rec3 != nullptr) {
       // The pullback reads lambda after the last odd half step of step i + 1
@@ -676,7 +686,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
         for (int m = 0; m < NC; ++m) l1[m] = l0[m];
       }
 #pragma unroll
-      for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, l1[m]);
+      for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, l1[m]);
       dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
       return;
     }
@@ -694,7 +704,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
     for (int m = 0; m < NC; ++m) l1[m] = l0[m];
   }
 #pragma unroll
-  for (int m = 0; m < NC; ++m) stream_store(out + m * V + p, l1[m]);
+  for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, l1[m]);
 }
 
 template <int N, int DIM>
