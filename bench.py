@@ -1,19 +1,26 @@
 """Benchmark of the PN-CSD forward+adjoint hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serve|raster2d_grad] [--requests R]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serve|raster2d_grad|raster3d_step|kernel]
 
-serve (default): the mixed serving stream -- R forward+gradient requests (PN
-order 3-9, 1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements) with
-Poisson arrivals, served by the continuous-batching server under an HBM
-budget.  A "step" is one pass of the whole stream; the timed region covers K
-passes with all request inputs prepared in host memory beforehand.  Reported
-beside it: p50/p99 latency (arrival to result), the plain one-request-at-a-time
-GPU loop on a sample of the same requests, and the C oracle on one core.
-The timed passes offer 1.5x the capacity measured on the head of the stream,
-so the server stays saturated and the throughput is its own; p50/p99 come
-from an untimed pass at 0.8x the measured throughput.
+serve (default): the mixed serving stream -- R = 4096 forward+gradient requests
+(PN order 3-9, 1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements,
+stable step counts) with Poisson arrivals, served by the continuous-batching
+server under an HBM budget.  The timed region is ONE continuous pass of the
+whole stream; a "step" is one window of R/K requests of it, so ms_per_step is
+the pass's wall time over K.  The W warm-up windows are the head of the stream
+served untimed (all at once; this also measures the capacity from which the
+arrival rate of the timed pass is set, 1.5x, so the server stays saturated and
+`value` is its own throughput).  Request inputs are prepared in host memory
+before the timed region; their preparation time is reported beside `value`.
+Reported with it: p50/p99 latency from an untimed pass of the first half of
+the stream at 0.8x the measured throughput, the plain one-request-at-a-time
+GPU loop (same kernels, taped like the server, no batching), a stratified
+sample of 64 served results checked against the C oracle (intensities 1e-10,
+density gradients 1e-8), and the C oracle's CPU throughput on the box's host
+cores.
 
 raster2d_grad: 2D P7 128x96, 32 beams, forward+adjoint of the batch per step.
+raster3d_step: 3D P5 128x128x96, one beam, a sample of its energy steps.
 
 Every mode also measures the roofline figure of the dominant kernel (the fused
 even-parity half step of the 2D P7 128x96 batch) with HIP events on the
@@ -45,20 +52,36 @@ METRIC = "served fwd+adj solves/s, p50/p99 latency vs plain GPU loop @ k-ratio e
 
 def _prep_one(spec):
     from pnepma_amd import serving
-    return serving.prepare(spec)
+    t = time.perf_counter()
+    pr = serving.prepare(spec)
+    return pr, time.perf_counter() - t
+
+
+WORKERS = 16  # the host cores of one GPU's share of the box
+
+
+def host_workers():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(WORKERS, n))
 
 
 def prepare_stream(n, seed=3):
-    """Draw and prepare the request stream on the host (process pool, before the GPU is touched)."""
+    """Draw and prepare the request stream on the host (process pool, before the
+    GPU is touched).  Returns the specs, the prepared requests, the mean
+    preparation time of one request on one core and the pool's wall time."""
     import multiprocessing as mp
     from pnepma_amd import serving, physics
     for e in physics.ELEMENTS.values():
         physics.transport_coefficients(e, 1000.0, 9)
     specs = serving.serving_requests(n, seed=seed)
-    workers = max(1, min(16, (os.cpu_count() or 1)))
-    with mp.get_context("fork").Pool(workers) as pool:
-        preps = pool.map(_prep_one, specs, chunksize=8)
-    return specs, preps
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(host_workers()) as pool:
+        out = pool.map(_prep_one, specs, chunksize=8)
+    wall = time.perf_counter() - t
+    return specs, [o[0] for o in out], float(sum(o[1] for o in out) / len(out)), wall
 
 
 PROBE_BEAMS = 160  # 160 beams x 3.57 MB of state: the working set exceeds the 256 MB Infinity Cache
@@ -105,32 +128,27 @@ def roofline_probe(reps=50):
                         "algorithmic_bytes": by_r, "bound": "latency (occupancy 3)"}}
 
 
-def serve_run(preps, rate, budget, max_active, tape_fraction=None):
-    """Submit every prepared request with Poisson arrivals of the given rate; returns
-    (wall seconds from first arrival to last completion, latencies in seconds, stats)."""
+def serve_run(srv, preps, rate):
+    """Submit every prepared request to the running server with Poisson arrivals
+    of the given rate (0: all at once) and wait for all of them; returns (wall
+    seconds from the first arrival to the last completion, latencies in
+    seconds, requests admitted taped)."""
     import numpy as np
-    from pnepma_amd import serving
     rng = np.random.default_rng(7)
     gaps = rng.exponential(1.0 / rate, len(preps)) if rate > 0 else np.zeros(len(preps))
     arr = np.cumsum(gaps)
     arr -= arr[0]
-    srv = serving.Server(budget, max_active=max_active)
-    if tape_fraction is not None:
-        srv.set_tape_limit(int(tape_fraction * budget))
     t0 = srv.clock() + 0.05
     tickets = [srv.submit(pr, arrival_s=t0 + a) for pr, a in zip(preps, arr)]
     progress(f"submitted {len(tickets)} requests in {srv.clock() - t0 + 0.05:.3f} s")
     srv.drain()
     tm = [srv.wait(t) for t in tickets]
-    stats = srv.stats()
-    stats["taped"] = sum(x.taped for x in tm)
-    srv.close()
     lat = np.array([x.done_s - x.arrival_s for x in tm])
     wall = max(x.done_s for x in tm) - t0
     bad = sum(1 for x in tm if x.status != 0)
     if bad:
         raise RuntimeError(f"{bad} served requests failed")
-    return wall, lat, stats
+    return wall, lat, sum(x.taped for x in tm)
 
 
 def latency_split(specs, lat, passes):
@@ -145,17 +163,21 @@ def latency_split(specs, lat, passes):
     return out
 
 
-def sequential_run(specs, preps, max_seconds=20.0, max_requests=64):
-    """The plain loop: one request at a time with the same kernels, no batching, no graph."""
+def sequential_run(specs, preps, max_seconds=15.0, max_requests=64):
+    """The plain loop: one request at a time with the same kernels, no batching.
+    Each request is taped like a served one (its whole forward march kept, no
+    recompute: checkpoint interval = step count), so the ratio to the served
+    rate measures batching alone.  Only forward_adjoint on a prepared device
+    problem is timed."""
     import numpy as np
     from pnepma_amd.solver import DeviceProblem
     times, outs = [], []
     t_all = time.perf_counter()
     for spec, pr in zip(specs, preps):
-        K = max(1, math.ceil(math.sqrt(spec.problem.n_steps)))
-        dp = DeviceProblem(spec.problem, pr.si, ckpt_every=K)
+        dp = DeviceProblem(spec.problem, pr.si, ckpt_every=spec.problem.n_steps)
+        dp.reserve_adjoint()  # the server's arena is allocated once too
         t = time.perf_counter()
-        I, _, _ = dp.forward_adjoint(spec.I_bar[None, :])
+        I, _, _ = dp.forward_adjoint(spec.I_bar[None, :], want_rho_bar=False)
         times.append(time.perf_counter() - t)
         outs.append(I[0])
         dp.close()
@@ -184,77 +206,139 @@ def request_cost(problem):
     return problem.n_steps * (fwd + rev)
 
 
-def cpu_sample(specs, preps, budget_s=24.0, candidates=256):
-    """The C oracle (single core) on a bounded sample of the stream, per stratum
-    (1D requests get a fifth of the budget, 2D/3D the rest, as the latter carry
-    nearly all the bytes).  Within a stratum requests are taken in stream order
-    from the head, skipping any whose predicted time (its cost times the
-    seconds per cost unit measured so far) would overrun the stratum's budget;
-    the first one taken is the first of at most the stratum's median cost.
-    The whole stream's oracle time is then predicted from each request's cost
-    and its stratum's measured seconds per cost unit.  Returns the CPU
-    throughput in solves/s, the sample size, its seconds and the sampled
-    request indices."""
+def check_sample(specs, per_stratum=8):
+    """The served results checked against the oracle: the per_stratum cheapest
+    requests (algorithmic bytes) of every (PN order, dimension) stratum."""
+    import numpy as np
+    cost = np.array([request_cost(s.problem) for s in specs])
+    key = [(s.problem.N, s.problem.gspec.axes[1].M > 1) for s in specs]
+    out = []
+    for k in sorted(set(key)):
+        idx = [i for i in range(len(specs)) if key[i] == k]
+        out += sorted(idx, key=lambda i: cost[i])[:per_stratum]
+    return out
+
+
+_CPU = None
+
+
+def _oracle_one(i):
+    """Oracle forward+adjoint of request i (in a pool worker): (i, I, rho_bar, ac_bar, seconds)."""
+    specs, preps = _CPU
+    from oracle import oracle
+    t = time.perf_counter()
+    I, rb, acb, _ = oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
+    return i, I, rb, acb, time.perf_counter() - t
+
+
+def _median_time(fn, reps=5):
+    import numpy as np
+    fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_phase(specs, preps, check_idx, tmpdir):
+    """Everything that runs the C oracle, before the GPU is touched:
+      * the 1D slab (P5, 200 cells, Cu/Ni, 300 stable steps) forward on one core:
+        median latency of one solve and of its k-ratios (sample + two standards);
+      * one-core seconds per algorithmic byte of the stream's 1D and 2D requests,
+        on a subset of the check sample run alone;
+      * the whole check sample as concurrent solves on every host core of the
+        share (one process each), whose outputs also check the served results.
+    The stream's CPU throughput is its request count over the predicted time of
+    all its requests (each request's bytes times its dimension's measured
+    seconds per byte), on one core and on all of them."""
+    import multiprocessing as mp
     import numpy as np
     from oracle import oracle
+    from pnepma_amd import workloads
+    from pnepma_amd.problem import solve_inputs
+    global _CPU
+    oracle.use_native(tmpdir)
+    out = {"model": host_cpu_model(), "nproc": os.cpu_count(), "workers": host_workers()}
+    p, mat, stds, det = workloads.slab1d(n_steps=300)
+    si = solve_inputs(p, mat, det)
+    sis = [solve_inputs(p, m, det) for m in stds]
+    p.records()
+    out["slab1d_forward_ms"] = 1e3 * _median_time(lambda: oracle.forward(p, si))
+    out["slab1d_kratios_ms"] = 1e3 * _median_time(
+        lambda: [oracle.forward(p, si)] + [oracle.forward(p, x) for x in sis])
     cost = np.array([request_cost(s.problem) for s in specs])
-    multi = np.array([s.problem.gspec.axes[1].M > 1 for s in specs])
-    head = np.arange(min(candidates, len(specs)))
-    sec_per_cost, n_all, el_all, taken = {}, 0, 0.0, []
-    for stratum, share in ((False, 0.2), (True, 0.8)):
-        idx = head[multi[head] == stratum]
-        if idx.size == 0:
+    is2d = np.array([s.problem.gspec.axes[1].M > 1 for s in specs])
+    # one core alone: the cheapest 1D request of every order and the two
+    # cheapest 2D requests of every order (about 15 s)
+    alone = {False: [0.0, 0.0], True: [0.0, 0.0]}
+    picked = {}
+    for i in check_idx:
+        N, d2 = specs[i].problem.N, bool(is2d[i])
+        if picked.get((N, d2), 0) >= (2 if d2 else 1):
             continue
-        b, med = share * budget_s, float(np.median(cost[idx]))
-        el, done, n = 0.0, 0.0, 0
-        for i in idx:
-            if n == 0 and cost[i] > med:
-                continue
-            if n > 0 and el + cost[i] * el / done > b:
-                continue
-            t = time.perf_counter()
-            oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
-            el += time.perf_counter() - t
-            done += cost[i]
-            n += 1
-            taken.append(int(i))
-            if el > 0.75 * b:
-                break
-        sec_per_cost[stratum] = el / done
-        n_all += n
-        el_all += el
-    if not sec_per_cost:
-        return 0.0, 0, 0.0, taken
-    spc = np.array([sec_per_cost.get(bool(m), next(iter(sec_per_cost.values()))) for m in multi])
-    return len(specs) / float(np.sum(cost * spc)), n_all, el_all, taken
-
-
-_CPU_SAMPLE = None
-
-
-def _run_sample(_):
-    specs, preps, idx = _CPU_SAMPLE
-    from oracle import oracle
-    t = time.perf_counter()
-    for i in idx:
+        picked[(N, d2)] = picked.get((N, d2), 0) + 1
+        t = time.perf_counter()
         oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
-    return time.perf_counter() - t
-
-
-def cpu_all_cores(specs, preps, idx, single_s, workers):
-    """The same oracle sample run as `workers` concurrent copies, one process
-    each (before the GPU is touched).  Every copy takes longer than the one-core
-    run when the cores contend for memory; the all-core throughput is the
-    one-core throughput times workers over that slowdown.  Returns (slowdown,
-    wall seconds)."""
-    import multiprocessing as mp
-    global _CPU_SAMPLE
-    _CPU_SAMPLE = (specs, preps, idx)
+        alone[d2][0] += time.perf_counter() - t
+        alone[d2][1] += cost[i]
+    spc1 = {k: v[0] / v[1] for k, v in alone.items() if v[1] > 0}
+    # every core: the check sample, longest first
+    _CPU = (specs, preps)
+    order = sorted(check_idx, key=lambda i: -cost[i])
     t = time.perf_counter()
-    with mp.get_context("fork").Pool(workers) as pool:
-        times = pool.map(_run_sample, range(workers), chunksize=1)
-    wall = time.perf_counter() - t
-    return sum(times) / len(times) / single_s, wall
+    with mp.get_context("fork").Pool(out["workers"]) as pool:
+        res = pool.map(_oracle_one, order, chunksize=1)
+    wall_all = time.perf_counter() - t
+    _CPU = None
+    busy = {False: [0.0, 0.0], True: [0.0, 0.0]}
+    for i, _, _, _, el in res:
+        busy[bool(is2d[i])][0] += el
+        busy[bool(is2d[i])][1] += cost[i]
+    spcn = {k: v[0] / v[1] for k, v in busy.items() if v[1] > 0}
+    t1 = float(sum(cost[i] * spc1[bool(is2d[i])] for i in range(len(specs))))
+    tn = float(sum(cost[i] * spcn[bool(is2d[i])] for i in range(len(specs))))
+    out["one_core_rate"] = len(specs) / t1
+    out["all_core_rate"] = len(specs) / tn * out["workers"]
+    out["one_core_sample"] = {"requests": sum(picked.values()), "seconds": float(sum(v[0] for v in alone.values()))}
+    out["all_core_sample"] = {"requests": len(res), "wall_s": wall_all,
+                              "busy_s": float(sum(v[0] for v in busy.values()))}
+    out["contention"] = {("2d" if k else "1d"): spcn[k] / spc1[k] for k in spc1 if k in spcn}
+    return out, {r[0]: r for r in res}
+
+
+def served_vs_oracle(specs, preps, ref):
+    """Largest relative differences of the served results from the oracle's:
+    intensities, density gradients of every updated class, absorption-factor
+    gradients (from the generation sums)."""
+    import numpy as np
+
+    def rel(a, b):
+        sc = float(np.max(np.abs(b)))
+        return float(np.max(np.abs(a - b))) / sc if sc > 0 else float(np.max(np.abs(a)))
+    eI = erb = eac = 0.0
+    for i, (_, I, rb, acb, _) in ref.items():
+        pr, p = preps[i], specs[i].problem
+        eI = max(eI, rel(pr.I_out, I))
+        for c in p.gspec.active_classes():
+            if rb[c].size:
+                erb = max(erb, rel(pr.rho_bar[c], rb[c]))
+        eac = max(eac, rel(pr.si.noa[:, pr.si.el_idx] * pr.gen, acb))
+    return {"requests": len(ref), "max_rel_err_intensity": eI, "max_rel_err_rho_bar": erb,
+            "max_rel_err_ac_bar": eac, "tolerance": {"intensity": 1e-10, "gradients": 1e-8},
+            "pass": bool(eI <= 1e-10 and erb <= 1e-8 and eac <= 1e-8),
+            "sample": "the 8 cheapest requests of each (PN order 3/5/7/9, 1D/2D) stratum of the stream"}
 
 
 def progress(msg):
@@ -262,16 +346,22 @@ def progress(msg):
 
 
 def main_serve(args):
+    import tempfile
     import numpy as np
-    specs, preps = prepare_stream(args.requests)
-    progress(f"prepared {len(preps)} requests")
-    # CPU baselines first, while no process has touched the GPU
-    cpu_rate, n_cpu, el_cpu, cpu_idx = cpu_sample(specs, preps) if not args.no_cpu else (0.0, 0, 0.0, [])
-    progress(f"cpu sample: {n_cpu} requests in {el_cpu:.1f} s")
-    workers = max(1, min(16, os.cpu_count() or 1))
-    if n_cpu:
-        slowdown, wall_all = cpu_all_cores(specs, preps, cpu_idx, el_cpu, workers)
-        progress(f"cpu sample x{workers} processes: {wall_all:.1f} s, slowdown per copy {slowdown:.2f}")
+    t_start = time.perf_counter()
+    R, K, W = args.requests, max(1, args.steps), max(0, args.warmup)
+    specs, preps, prep_s, prep_wall = prepare_stream(R)
+    progress(f"prepared {len(preps)} requests: {prep_s * 1e3:.1f} ms each on one core, {prep_wall:.1f} s wall")
+    check_idx = check_sample(specs) if not args.no_cpu else []
+    for i in check_idx:
+        preps[i].want_gradients()
+    cpu, ref = None, {}
+    if not args.no_cpu:
+        # the C oracle runs before any process touches the GPU
+        with tempfile.TemporaryDirectory() as tmp:
+            cpu, ref = cpu_phase(specs, preps, check_idx, tmp)
+        progress(f"cpu phase: one core {cpu['one_core_rate']:.4f} solves/s, {cpu['workers']} cores "
+                 f"{cpu['all_core_rate']:.3f} solves/s; check sample {cpu['all_core_sample']['wall_s']:.1f} s wall")
     import torch
     assert torch.cuda.is_available()
     torch.cuda.init()
@@ -279,81 +369,101 @@ def main_serve(args):
     from pnepma_amd import serving
     free, total = torch.cuda.mem_get_info()
     budget = int(min(free * 0.9, 270e9))
-    max_active = 4096
-    # capacity probe on the head of the stream, all arriving at once; its tail
-    # is poorly batched, so it underestimates the sustained capacity
-    head = preps[:min(len(preps), 512)]
-    w0, _, _ = serve_run(head, 0.0, budget, max_active, args.tape_fraction)
-    capacity = len(head) / w0
-    progress(f"capacity probe: {len(head)} requests in {w0:.2f} s")
-    # the timed passes offer more than the server sustains, so it stays saturated
-    # and `value` is its throughput, not the arrival rate
+    # one server for every pass: its arena is allocated once, outside the timed region
+    srv = serving.Server(budget, max_active=4096)
+    if args.tape_fraction is not None:
+        srv.set_tape_limit(int(args.tape_fraction * budget))
+    # warm-up: the head of the stream, W windows of R/K requests (at least 512)
+    # all at once; its rate also sets the arrival rate of the timed pass
+    n_head = min(R, max(512, W * R // K))
+    w0, _, _ = serve_run(srv, preps[:n_head], 0.0)
+    capacity = n_head / w0
     rate = 1.5 * capacity
-    for _ in range(max(0, args.warmup - 1)):
-        serve_run(preps[:64], rate, budget, max_active, args.tape_fraction)
+    progress(f"warm-up / capacity probe: {n_head} requests in {w0:.2f} s")
+    st0 = srv.stats()
+    # timed: one continuous pass of the whole stream (K windows of R/K requests)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    walls, lats, stats = [], [], None
-    for _ in range(args.steps):
-        w, lat, stats = serve_run(preps, rate, budget, max_active, args.tape_fraction)
-        walls.append(w)
-        lats.append(lat)
-        progress(f"served pass: {len(preps)} requests in {w:.2f} s")
+    w, lat_sat, taped = serve_run(srv, preps, rate)
     torch.cuda.synchronize()
     total_wall = time.perf_counter() - t0
-    lat_over = np.concatenate(lats)
-    served = len(preps) * args.steps / sum(walls)
-    # latency at a load the server sustains: the same stream at 80 % of the
-    # measured throughput (untimed)
+    st1 = srv.stats()
+    stats = {"ticks": st1["ticks"] - st0["ticks"], "completed": st1["completed"] - st0["completed"],
+             "peak_active": st1["peak_active"], "peak_bytes": st1["peak_bytes"], "taped": taped}
+    served = R / total_wall
+    progress(f"timed pass: {R} requests in {total_wall:.2f} s ({served:.1f} solves/s; server {w:.2f} s)")
+    check = served_vs_oracle(specs, preps, ref) if ref else None
+    if check:
+        progress(f"served vs oracle ({check['requests']} requests): I {check['max_rel_err_intensity']:.2e} "
+                 f"rho_bar {check['max_rel_err_rho_bar']:.2e} ac_bar {check['max_rel_err_ac_bar']:.2e}")
+    # latency at a load the server sustains: the first half of the stream at
+    # 80 % of the measured throughput (untimed)
+    n_lat = R // 2
     rate_lat = 0.8 * served
-    w_lat, lat, stats_lat = serve_run(preps, rate_lat, budget, max_active, args.tape_fraction)
-    progress(f"latency pass at {rate_lat:.1f}/s: {len(preps)} requests in {w_lat:.2f} s, "
+    w_lat, lat, taped_lat = serve_run(srv, preps[:n_lat], rate_lat)
+    srv.close()
+    progress(f"latency pass at {rate_lat:.1f}/s: {n_lat} requests in {w_lat:.2f} s, "
              f"p50 {np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms")
-    # results of the served pass vs the same requests alone (plain loop)
     seq_t, seq_I = sequential_run(specs, preps)
     seq_rate = len(seq_t) / seq_t.sum()
     progress(f"plain loop: {len(seq_t)} requests, {seq_rate:.2f} solves/s")
-    # the plain-loop sample is the head of the stream: compare with the served results
     alone_err = max(float(np.max(np.abs(a - pr.I_out)) / np.max(np.abs(a))) for a, pr in zip(seq_I, preps))
     roof = roofline_probe()
+    progress(f"roofline probes done; {time.perf_counter() - t_start:.0f} s since start")
     out = {
-        "metric": METRIC, "value": served, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * total_wall / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"serving stream of {len(preps)} forward+gradient requests: PN 3/5/7/9, "
-                               "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, CFL-0.5 step counts, "
-                               "Poisson arrivals at 1.5x the all-at-once capacity of its first 512 requests",
-                   "requests": len(preps), "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
+        "metric": METRIC, "value": served, "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 * total_wall / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serving stream of {R} forward+gradient requests: PN 3/5/7/9, "
+                               "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, stable (CFL 0.5) step "
+                               "counts, Poisson arrivals at 1.5x the all-at-once capacity of the stream's head; "
+                               f"one timed pass, a step = one window of {R // K} requests",
+                   "requests": R, "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
                    "tape_limit": "0.95 of the budget (server default)" if args.tape_fraction is None
                    else f"{args.tape_fraction} of the budget",
                    "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
+        "server_wall_s": w,
+        "host_prep": {"ms_per_request_one_core": prep_s * 1e3, "pool_wall_s": prep_wall, "workers": host_workers(),
+                      "note": "per-request host preparation (densities, absorption ray-march, records, beam "
+                              "moments), outside the timed region"},
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "latency_load": {"arrival_rate_per_s": rate_lat, "note": "p50/p99 from an untimed pass of the same stream "
-                         "at 0.8x the measured throughput", "taped": stats_lat["taped"]},
-        "latency_ms_by_grid": latency_split(specs, lat, 1),
-        "saturated_pass_latency_ms": {"p50": float(np.percentile(lat_over, 50) * 1e3),
-                                      "p99": float(np.percentile(lat_over, 99) * 1e3)},
-        "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "sample": f"first {len(seq_t)} requests of "
-                           "the stream, one at a time, same kernels, no batching, no graph",
+        "latency_load": {"arrival_rate_per_s": rate_lat, "requests": n_lat, "taped": taped_lat,
+                         "note": "p50/p99 from an untimed pass of the first half of the stream at 0.8x the "
+                                 "measured throughput"},
+        "latency_ms_by_grid": latency_split(specs[:n_lat], lat, 1),
+        "saturated_pass_latency_ms": {"p50": float(np.percentile(lat_sat, 50) * 1e3),
+                                      "p99": float(np.percentile(lat_sat, 99) * 1e3)},
+        "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "taped": True,
+                           "sample": f"first {len(seq_t)} requests of the stream, one at a time, same kernels, "
+                                     "taped like the server, no batching",
                            "mean_ms": float(seq_t.mean() * 1e3)},
         "speedup_vs_plain_gpu_loop": served / seq_rate,
         "served_vs_alone_max_rel_err": alone_err,
+        "served_vs_oracle": check,
         "server_stats": stats,
         "roofline": roof,
     }
-    if n_cpu:
-        out["cpu_baseline"] = {"value": cpu_rate, "unit": "solves/s", "cores": 1, "kind": "port",
-                               "sample": f"C oracle forward+adjoint of {n_cpu} 1D and 2D requests from the head of "
-                                         f"the stream ({el_cpu:.1f} s on one core); the whole stream's oracle time "
-                                         "is predicted from each request's algorithmic bytes (bench.request_cost) "
-                                         "and its dimension's measured seconds per byte"}
-        out["speedup_vs_cpu"] = served / cpu_rate
-        all_rate = cpu_rate * workers / slowdown
-        out["cpu_baseline_all_cores"] = {"value": all_rate, "unit": "solves/s", "cores": workers, "kind": "port",
-                                         "sample": f"the same sample as {workers} concurrent processes "
-                                                   f"({wall_all:.1f} s wall; each copy {slowdown:.2f}x its "
-                                                   "one-core time); rate = one-core rate x cores / slowdown"}
-        out["speedup_vs_cpu_all_cores"] = served / all_rate
+    if cpu:
+        out["cpu_baseline"] = {
+            "value": cpu["one_core_rate"], "unit": "solves/s", "cores": 1, "kind": "port",
+            "host": {"model": cpu["model"], "nproc": cpu["nproc"]},
+            "sample": f"C oracle (gcc -O3 -march=native) forward+adjoint of {cpu['one_core_sample']['requests']} "
+                      f"requests of the stream alone on one core ({cpu['one_core_sample']['seconds']:.1f} s); the "
+                      "stream's time is predicted from each request's algorithmic bytes (request_cost) and its "
+                      "dimension's measured seconds per byte"}
+        out["speedup_vs_cpu"] = served / cpu["one_core_rate"]
+        out["cpu_baseline_all_cores"] = {
+            "value": cpu["all_core_rate"], "unit": "solves/s", "cores": cpu["workers"], "kind": "port",
+            "sample": f"the {cpu['all_core_sample']['requests']} requests of the oracle check as concurrent "
+                      f"solves, one process per core ({cpu['all_core_sample']['wall_s']:.1f} s wall); "
+                      "seconds per byte under contention: " +
+                      ", ".join(f"{k} x{v:.2f}" for k, v in cpu["contention"].items()),
+        }
+        out["speedup_vs_cpu_all_cores"] = served / cpu["all_core_rate"]
+        out["cpu_slab1d_p5_200cells"] = {"forward_ms": cpu["slab1d_forward_ms"],
+                                         "kratios_ms": cpu["slab1d_kratios_ms"], "cores": 1,
+                                         "note": "C oracle, 300 stable steps, median of 5 after one warm-up; "
+                                                 "k-ratios = the sample and the two pure standards"}
     print(json.dumps(out), flush=True)
 
 
@@ -472,7 +582,7 @@ def main_kernel(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "raster3d_step", "kernel"])
     ap.add_argument("--requests", type=int, default=4096)

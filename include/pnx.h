@@ -32,7 +32,8 @@ enum {
   PNX_ERR_DEVICE = -2,   /* HIP runtime error */
   PNX_ERR_MEMORY = -3,   /* device allocation failed or budget exceeded */
   PNX_ERR_STATE = -4,    /* call out of order (e.g. solve before inputs are set) */
-  PNX_ERR_UNSUPPORTED = -5
+  PNX_ERR_UNSUPPORTED = -5,
+  PNX_ERR_CANCELLED = -6 /* pnx_timing.status of a served request cancelled before completion */
 };
 
 typedef struct pnx_context pnx_context;
@@ -51,7 +52,8 @@ typedef struct {
   int32_t n_beams;    /* beams in the batch */
   double E_initial;   /* eV */
   double E_cutoff;    /* eV */
-  int32_t ckpt_every; /* adjoint checkpoint interval in energy steps (0 = n_steps) */
+  int32_t ckpt_every; /* adjoint checkpoint interval in energy steps; 0 or >= n_steps: the whole
+                         forward march is kept as a tape and not recomputed */
 } pnx_desc;
 
 const char* pnx_last_error(void);
@@ -142,6 +144,13 @@ int pnx_forward(pnx_problem* p, double* I_out);
  *                          from which ac_bar = noa[el_r] * gen and noa_bar[el_r] += ac * gen
  *                          (x_ray_intensities.jl:193-223). May be NULL. */
 int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out);
+
+/* Allocates the adjoint sweep's device buffers (adjoint state, checkpoints,
+ * segment or tape, gradients) now rather than in the first
+ * pnx_forward_adjoint (exact = 0) or pnx_forward_adjoint_exact (exact = 1).
+ * With ckpt_every >= n_steps the whole forward march is kept (a tape of
+ * n_steps states) and the reverse sweep needs no recompute. */
+int pnx_problem_reserve_adjoint(pnx_problem* p, int exact);
 
 /* Exact gradient of sum_{b,r} I_bar[b][r] I[b][r] of the discrete forward
  * march (the transpose of every half step, including the exponential factor
@@ -235,11 +244,19 @@ double pnx_server_clock(const pnx_server* s);
 int pnx_server_set_tape_limit(pnx_server* s, int64_t bytes);
 int pnx_server_submit(pnx_server* s, const pnx_request* req, int64_t* ticket);
 int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t);
+/* Blocks until the request is complete; this releases the ticket (a later
+ * poll, wait or cancel of it returns PNX_ERR_ARG). */
 int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t);
+/* Cancels a request: a pending one is dropped at once, an admitted one issues
+ * no further passes and completes after its lane's launched work has drained.
+ * Its outputs are not written and its pnx_timing.status is PNX_ERR_CANCELLED.
+ * A request that has already completed is left as it is.  The reference has no
+ * counterpart (it runs one blocking solve per call). */
+int pnx_server_cancel(pnx_server* s, int64_t ticket);
 /* Block until every submitted request has completed. */
 int pnx_server_drain(pnx_server* s);
 /* ticks, completed requests, peak concurrently admitted requests, peak device bytes in use */
-int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active, int64_t* peak_bytes);
+int pnx_server_stats(pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active, int64_t* peak_bytes);
 /* Device bytes a request would hold while admitted. */
 int64_t pnx_request_device_bytes(const pnx_desc* desc);
 /* The operation sequence the server runs for a request with n energy steps and

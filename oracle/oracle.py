@@ -36,14 +36,29 @@ def build(force: bool = False) -> str:
 
 
 _lib = None
+_path = LIB
+
+
+def use_native(out_dir: str) -> str:
+    """Builds a copy of the oracle for the host it runs on (-O3 -march=native)
+    into out_dir and loads that copy instead of the committed build (which
+    targets x86-64-v3 so that it runs on any host it is copied to).  Used by the
+    benchmark's CPU baseline."""
+    global _lib, _path
+    out = os.path.join(out_dir, "libpnx_oracle_native.so")
+    subprocess.check_call(["gcc", "-O3", "-march=native", "-std=c99", "-fPIC", "-shared", "-o", out,
+                           os.path.join(HERE, "pnx_ref.c"), "-lm"])
+    _lib, _path = None, out
+    lib()
+    return out
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if _path == LIB and not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(_path)
         L.ref_forward.argtypes = [C.POINTER(Desc), C.POINTER(Inputs), _dp, _dp]
         L.ref_forward_adjoint.argtypes = [C.POINTER(Desc), C.POINTER(Inputs), _dp, _dp, _dp, _dp, _dp,
                                           _dp * 8, _dp, _dp]

@@ -52,6 +52,7 @@ SIGNATURES = {
     "pnx_problem_material_gradient": (C.c_int, [C.c_void_p, _dp]),
     "pnx_march_sequence": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
     "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
+    "pnx_problem_reserve_adjoint": (C.c_int, [C.c_void_p, C.c_int]),
     "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
     "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
     "pnx_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp * 8]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "pnx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
     "pnx_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(Timing)]),
     "pnx_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Timing)]),
+    "pnx_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),
     "pnx_server_drain": (C.c_int, [C.c_void_p]),
     "pnx_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64)]),
@@ -76,6 +78,9 @@ SIGNATURES = {
 }
 
 _lib = None
+
+
+PNX_ERR_CANCELLED = -6
 
 
 class PnxError(RuntimeError):

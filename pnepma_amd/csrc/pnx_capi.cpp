@@ -82,6 +82,12 @@ struct pnx_problem {
   long abs_off_mac = 0, abs_off_am = 0, abs_off_emb = 0;
   bool have_cells = false, have_abs = false, have_rho_bar = false;
   hipGraphExec_t fwd_graph = nullptr;
+  // The problem's own stream: its launches, copies and graph capture never
+  // share a stream with a server attached to the same context.
+  hipStream_t stream = nullptr;
+  ~pnx_problem() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 static void destroy_graph(pnx_problem* p) {
@@ -163,6 +169,11 @@ int pnx_problem_create(pnx_context* ctx, const pnx_desc* desc, pnx_problem** out
   auto* p = new pnx_problem;
   p->ctx = ctx;
   p->d = d;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    p->stream = nullptr;
+    delete p;
+    return fail(PNX_ERR_DEVICE, "hipStreamCreate failed");
+  }
   std::string err;
   if (int e = make_geometry(d, p->g, err)) {
     delete p;
@@ -203,7 +214,7 @@ int pnx_problem_create(pnx_context* ctx, const pnx_desc* desc, pnx_problem** out
 int pnx_problem_destroy(pnx_problem* p) {
   if (!p) return PNX_OK;
   (void)hipSetDevice(p->ctx->device);
-  (void)hipStreamSynchronize(p->ctx->stream);
+  (void)hipStreamSynchronize(p->stream);
   destroy_graph(p);
   if (p->line_el) (void)hipFree(p->line_el);
   delete p;
@@ -215,9 +226,10 @@ int64_t pnx_problem_device_bytes(const pnx_problem* p) {
   size_t n = 0;
   for (const DevBuf* b : {&p->state, &p->rho, &p->W, &p->coef_f, &p->coef_a, &p->rec_f, &p->rec_a, &p->rec_d,
                           &p->prof, &p->mms, &p->partial, &p->I_dev, &p->lam, &p->ckpt, &p->seg, &p->rho_bar_b,
-                          &p->gen_b, &p->ibar, &p->sum_buf, &p->w1, &p->w2})
+                          &p->gen_b, &p->ibar, &p->sum_buf, &p->w1, &p->w2, &p->cell_w, &p->rho_pure, &p->extra,
+                          &p->grad, &p->abs_small, &p->mu, &p->ac})
     n += b->n;
-  return (int64_t)(n * sizeof(double));
+  return (int64_t)(n * sizeof(double) + (p->line_el ? 8 * sizeof(int) : 0));
 }
 
 int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj, const double* dens) {
@@ -225,7 +237,7 @@ int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj
   if ((adj == nullptr) != (dens == nullptr)) return fail(PNX_ERR_ARG, "adj and dens must be given together");
   HIP_TRY(hipSetDevice(p->ctx->device));
   const size_t n = (size_t)(p->d.n_steps + 1) * p->g.rec_len;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->rec_f.p, fwd, n * sizeof(double), hipMemcpyHostToDevice, s));
   if (adj) {
     int rc;
@@ -251,7 +263,7 @@ int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const d
   }
   std::vector<double> w((size_t)p->d.n_lines * g.L.V, 0.0);
   pack_lines(g, W, w.data());
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->rho.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->W.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -269,7 +281,7 @@ int pnx_problem_set_cell_fractions(pnx_problem* p, const double* w, const double
   const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
   int rc;
   if ((rc = p->cell_w.alloc(n_cells * ne)) || (rc = p->rho_pure.alloc(ne))) return rc;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->cell_w.p, w, n_cells * ne * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->rho_pure.p, rho_pure, ne * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(launch_fill(p->rho.p, g.rho_total, 0.0, s));
@@ -292,7 +304,7 @@ int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, 
   const Geometry& g = p->g;
   const int ne = p->d.n_elements;
   const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   int rc;
   if ((rc = p->grad.alloc(n_cells * ne))) return rc;
   std::vector<double> h;
@@ -348,7 +360,7 @@ int pnx_problem_set_absorption(pnx_problem* p, const double* mac, const double* 
     p->line_el = nullptr;
     return fail(PNX_ERR_MEMORY, "hipMalloc failed");
   }
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->abs_small.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->line_el, line_element, nl * sizeof(int), hipMemcpyHostToDevice, s));
   AbsorptionGeometry& ag = p->abs;
@@ -387,7 +399,7 @@ int pnx_problem_material_gradient(pnx_problem* p, double* grad) {
   const Geometry& g = p->g;
   const int ne = p->d.n_elements;
   const size_t n_cells = (size_t)g.L.Mx * g.L.My * g.L.Mz;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   int rc;
   if ((rc = p->grad.alloc(n_cells * ne)) || (rc = p->extra.alloc((size_t)ne * g.L.V))) return rc;
   const long gs = (long)p->d.n_lines * g.L.V;
@@ -417,7 +429,7 @@ int pnx_problem_set_beams(pnx_problem* p, const double* const mms[8], const doub
       if (!ppos[b * 8 + c]) return fail(PNX_ERR_ARG, "beam profile missing for an x-odd class");
   }
   for (int b = 0; b < p->d.n_beams; ++b) pack_profiles(g, ppos + b * 8, pr.data() + b * g.prof_stride);
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->mms.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->prof.p, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -455,7 +467,7 @@ HalfStepArgs half_args(pnx_problem* p, int mode, const double* rec, double dt, c
 // One energy step of the forward problem (step_pde!): odd half, even full, odd half.
 // src -> dst (dst may equal src); partial receives the intensity partials of this step.
 int forward_step(pnx_problem* p, int k, const double* src, double* dst, double* partial) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const double* rec = p->rec_f.p + (long)k * p->g.rec_len;
   const int N = p->d.N, dims = p->L.dims;
   const double h = p->g.dE / 2;
@@ -473,7 +485,7 @@ int forward_step(pnx_problem* p, int k, const double* src, double* dst, double* 
 // partials of step k.
 template <class Dst, class Part>
 int forward_run(pnx_problem* p, int ka, int kb, const double* src0, Dst dst, Part partial) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const int N = p->d.N, dims = p->L.dims;
   const double h = p->g.dE / 2;
   auto rec = [&](int k) { return p->rec_f.p + (long)k * p->g.rec_len; };
@@ -497,7 +509,7 @@ int forward_run(pnx_problem* p, int ka, int kb, const double* src0, Dst dst, Par
 }
 
 int adjoint_step(pnx_problem* p, int i, double* lam) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const double* rec = p->rec_a.p + (long)i * p->g.rec_len;
   const int N = p->d.N, dims = p->L.dims;
   const double h = p->g.dE / 2;
@@ -508,7 +520,7 @@ int adjoint_step(pnx_problem* p, int i, double* lam) {
 }
 
 int enqueue_forward(pnx_problem* p) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(launch_fill(p->state.p, (long)p->L.beam_stride * p->d.n_beams, 0.0, s));
   double* st = p->state.p;
   int rc = forward_run(
@@ -523,7 +535,7 @@ int enqueue_forward(pnx_problem* p) {
 
 int ensure_graph(pnx_problem* p) {
   if (p->fwd_graph) return PNX_OK;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_forward(p);
@@ -609,15 +621,26 @@ RevArgs rev_args(pnx_problem* p, int i, const double* u, double int_fac, bool wi
 // Forward pass with checkpoints, then the reverse sweep of the reference
 // pullback (pn_intensities.jl:381-404).
 int enqueue_forward_adjoint(pnx_problem* p) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const long st = p->L.beam_stride * p->d.n_beams;
   const int n = p->d.n_steps, K = p->g.K, N = p->d.N, dims = p->L.dims;
   int rc;
   HIP_TRY(launch_fill(p->state.p, st, 0.0, s));
   double* state = p->state.p;
   auto part_k = [&](int k) { return p->partial.p + (long)k * p->g.partial_step; };
-  // one fused run per checkpoint segment: the state after each segment is stored
-  for (int sgi = 0; sgi < p->g.nck; ++sgi) {
+  // One segment (checkpoints every n steps or more): the forward run writes
+  // every state into the segment buffer, which the reverse sweep then reads
+  // as a tape, so the segment is not recomputed (2n + 1 sweeps instead of
+  // 3n + 1, with the same memory).
+  const bool taped = p->g.nck == 1;
+  if (taped) {
+    HIP_TRY(launch_fill(p->ckpt.p, st, 0.0, s));  // u_0 = 0 for the final pullback
+    if ((rc = forward_run(
+             p, 1, n, state, [&](int k) { return p->seg.p + (long)(k - 1) * st; }, part_k)))
+      return rc;
+  }
+  // otherwise one fused run per checkpoint segment: the state before each segment is stored
+  for (int sgi = 0; !taped && sgi < p->g.nck; ++sgi) {
     const int k0 = sgi * K, k1 = std::min(n, k0 + K);
     HIP_TRY(launch_copy(p->ckpt.p + (long)sgi * st, state, st, s));
     if ((rc = forward_run(p, k0 + 1, k1, state, [&](int) { return state; }, part_k))) return rc;
@@ -631,9 +654,9 @@ int enqueue_forward_adjoint(pnx_problem* p) {
     const int k0 = sgi * K, k1 = std::min(n, k0 + K);
     // recompute u_{k0+1} .. u_{k1} into the segment buffer (the partials go to
     // the scratch slot of step 0)
-    if ((rc = forward_run(
-             p, k0 + 1, k1, p->ckpt.p + (long)sgi * st, [&](int k) { return p->seg.p + (long)(k - k0 - 1) * st; },
-             [&](int) { return p->partial.p; })))
+    if (!taped && (rc = forward_run(
+                       p, k0 + 1, k1, p->ckpt.p + (long)sgi * st,
+                       [&](int k) { return p->seg.p + (long)(k - k0 - 1) * st; }, [&](int) { return p->partial.p; })))
       return rc;
     // per reverse step i: A3(i+1) + dens(i) + A1(i) fused, then AE(i)
     for (int i = k1; i > k0; --i) {
@@ -682,7 +705,7 @@ DiscArgs disc_args(pnx_problem* p, int k, const double* s_from, const double* s_
 // per segment, recompute the states after every step and after its first odd
 // half step, and back-propagate each step as three fused passes.
 int enqueue_forward_adjoint_exact(pnx_problem* p) {
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const long st = p->L.beam_stride * p->d.n_beams;
   const int n = p->d.n_steps, K = p->g.K, N = p->d.N, dims = p->L.dims;
   const double dE = p->g.dE, h = dE / 2;
@@ -740,7 +763,7 @@ int pnx_forward(pnx_problem* p, double* I_out) {
   if (rc) return rc;
   if (!I_out) return fail(PNX_ERR_ARG, "null output");
   if ((rc = ensure_graph(p))) return rc;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipGraphLaunch(p->fwd_graph, s));
   HIP_TRY(hipMemcpyAsync(I_out, p->I_dev.p, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -753,7 +776,7 @@ static int forward_adjoint_impl(pnx_problem* p, bool exact, const double* I_bar,
   if (rc) return rc;
   if (!I_bar || !I_out) return fail(PNX_ERR_ARG, "null argument");
   if ((rc = alloc_adjoint(p, exact))) return rc;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   HIP_TRY(hipMemcpyAsync(p->ibar.p, I_bar, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyHostToDevice, s));
   if ((rc = exact ? enqueue_forward_adjoint_exact(p) : enqueue_forward_adjoint(p))) return rc;
   p->have_rho_bar = true;
@@ -787,6 +810,12 @@ static int forward_adjoint_impl(pnx_problem* p, bool exact, const double* I_bar,
   return PNX_OK;
 }
 
+int pnx_problem_reserve_adjoint(pnx_problem* p, int exact) {
+  if (!p) return fail(PNX_ERR_ARG, "null problem");
+  if (hipSetDevice(p->ctx->device) != hipSuccess) return fail(PNX_ERR_DEVICE, "hipSetDevice failed");
+  return alloc_adjoint(p, exact != 0);
+}
+
 int pnx_forward_adjoint(pnx_problem* p, const double* I_bar, double* I_out, double* const rho_bar[8], double* gen_out) {
   return forward_adjoint_impl(p, false, I_bar, I_out, rho_bar, gen_out);
 }
@@ -802,7 +831,7 @@ int pnx_step(pnx_problem* p, int adjoint, int record, int beam, const double* I_
   if (!u || beam < 0 || beam >= p->d.n_beams || record < 0 || record > p->d.n_steps)
     return fail(PNX_ERR_ARG, "bad argument");
   const Layout& L = p->L;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   std::vector<double> h((size_t)L.beam_stride, 0.0);
   for (int c = 0; c < 8; ++c) {
     if (!p->g.active(c)) continue;
@@ -833,7 +862,7 @@ int pnx_forward_timed(pnx_problem* p, int reps, int use_graph, double* ms_out) {
   if (rc) return rc;
   if (reps < 1 || !ms_out) return fail(PNX_ERR_ARG, "bad argument");
   if (use_graph && (rc = ensure_graph(p))) return rc;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -860,7 +889,7 @@ int pnx_forward_adjoint_timed(pnx_problem* p, int reps, double* ms_out) {
   if (rc) return rc;
   if (reps < 1 || !ms_out) return fail(PNX_ERR_ARG, "bad argument");
   if ((rc = alloc_adjoint(p))) return rc;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -897,7 +926,7 @@ int pnx_rev_bench(pnx_problem* p, int reps, double* ms_out, double* bytes_out) {
   bytes *= p->d.n_beams;
   if (bytes_out) *bytes_out = bytes;
   if (!ms_out) return PNX_OK;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   // lambda, rho_bar, the generation sums and the seed are allocated by the
   // first adjoint solve; a problem that has not had one gets them here
   if ((rc = alloc_adjoint(p))) return rc;
@@ -948,7 +977,7 @@ int pnx_half_step_bench(pnx_problem* p, int parity, int reps, double* ms_out, do
   bytes *= p->d.n_beams;
   if (bytes_out) *bytes_out = bytes;
   if (!ms_out) return PNX_OK;
-  hipStream_t s = p->ctx->stream;
+  hipStream_t s = p->stream;
   const double* rec = p->rec_f.p + p->g.rec_len;
   const double dt = parity == 0 ? p->g.dE : p->g.dE / 2;
   HalfStepArgs a = half_args(p, kForward, rec, dt, p->state.p, p->state.p, p->state.p,

@@ -313,6 +313,7 @@ struct Request {
   MarchPos pos = march_start();
   int ops_left = 0;            // operations not yet issued
   int64_t done_tick = -1;      // lane tick that copied the results back
+  std::atomic<bool> cancel{false};  // set by pnx_server_cancel: issue nothing more
   pnx_timing timing{};
   bool done = false;
 };
@@ -746,7 +747,25 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   return 0;
 }
 
+// A request cancelled after admission: once its lane's stream has drained
+// (its upload and every launched pass read or write its memory) the memory is
+// returned; its outputs are left untouched.
+void finish_cancelled(pnx_server* s, Request& r) {
+  (void)hipStreamSynchronize(s->lanes[r.lane].stream);
+  s->pinned_pool.release(r.pinned, r.pinned_bytes);
+  r.pinned = nullptr;
+  r.pinned_out = nullptr;
+  s->arena.release(r.off, r.bytes);
+  r.timing.done_s = s->now();
+  r.timing.status = PNX_ERR_CANCELLED;
+  r.done = true;
+}
+
 void finish_request(pnx_server* s, Request& r) {
+  if (r.done_tick < 0) {  // cancelled before its results were copied back
+    finish_cancelled(s, r);
+    return;
+  }
   const long nI = r.d.n_lines, nrb = r.sz.rb;
   const double* out = r.pinned_out;
   if (r.I_out) std::memcpy(r.I_out, out, nI * sizeof(double));
@@ -794,7 +813,7 @@ int step_lane(pnx_server* s, int li) {
   bool any = false;
   if (li == 0) {
     for (auto& r : ln.active)
-      if (r->ops_left > 0) {
+      if (r->ops_left > 0 && !r->cancel) {
         plan_march(s, *r, plan);
         any = true;
       }
@@ -803,7 +822,7 @@ int step_lane(pnx_server* s, int li) {
       SubTick sub;
       bool here = false;
       for (auto& r : ln.active)
-        if (r->ops_left > 0) {
+        if (r->ops_left > 0 && !r->cancel) {
           plan_grid(s, *r, sub, plan);
           here = true;
         }
@@ -888,7 +907,8 @@ void worker_loop(pnx_server* s) {
       }
       // requests whose copy-out tick has completed
       auto it = std::partition(ln.active.begin(), ln.active.end(), [&](const std::shared_ptr<Request>& r) {
-        return !(r->done_tick >= 0 && r->done_tick <= ln.done_through);
+        if (r->done_tick >= 0) return !(r->done_tick <= ln.done_through);
+        return !r->cancel;  // cancelled before its results were copied back
       });
       finished.insert(finished.end(), it, ln.active.end());
       ln.active.erase(it, ln.active.end());
@@ -1114,6 +1134,37 @@ int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t) {
   s->done_cv.wait(lk, [&] { return r->done || s->error != 0; });
   if (t) *t = r->timing;
   if (s->error) return set_error(s->error, s->error_msg);
+  s->all.erase(ticket);  // the ticket is released: its request is complete
+  return PNX_OK;
+}
+
+int pnx_server_cancel(pnx_server* s, int64_t ticket) {
+  if (!s) return set_error(PNX_ERR_ARG, "null server");
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return set_error(PNX_ERR_ARG, "unknown ticket");
+    r = it->second;
+    if (r->done) return PNX_OK;
+    auto pit = std::find(s->pending.begin(), s->pending.end(), r);
+    if (pit != s->pending.end()) {
+      // not admitted: it holds only its pinned staging block
+      s->pending.erase(pit);
+      s->pinned_pool.release(r->pinned, r->pinned_bytes);
+      r->pinned = nullptr;
+      r->pinned_out = nullptr;
+      r->timing.done_s = s->now();
+      r->timing.status = PNX_ERR_CANCELLED;
+      r->done = true;
+    } else {
+      // admitted: the scheduler issues no more of its work and completes it
+      // once its launched ticks have passed
+      r->cancel = true;
+    }
+  }
+  s->done_cv.notify_all();
+  s->cv.notify_all();
   return PNX_OK;
 }
 
@@ -1125,9 +1176,10 @@ int pnx_server_drain(pnx_server* s) {
   return PNX_OK;
 }
 
-int pnx_server_stats(const pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active,
+int pnx_server_stats(pnx_server* s, int64_t* ticks, int64_t* completed, int64_t* peak_active,
                      int64_t* peak_bytes) {
   if (!s) return set_error(PNX_ERR_ARG, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
   if (ticks) *ticks = s->ticks;
   if (completed) *completed = s->completed;
   if (peak_active) *peak_active = s->peak_active;

@@ -51,17 +51,32 @@ def ko_range_um(E_keV, els, w):
     return 0.0276 * A * E_keV ** 1.67 / (Z ** 0.889 * rho)
 
 
-def cfl_steps(N, gspec, material, elements, E0, E1, cfl=0.5):
-    """Smallest step count with dE lambda_max sqrt(sum 1/h^2) / S_min <= cfl, where S_min is
-    the smallest rho . sp over the material at the highest coefficient energy."""
+def _cfl_x(N, gspec, material, elements, E0, E1):
+    """(E0 - E1) lambda_max sqrt(sum 1/h^2) / S_min: the stability number of a
+    one-step march over the whole energy range."""
     from .physics import StoppingPower
     h = [a.h for a in gspec.axes if not a.singleton]
     inv = math.sqrt(sum(1.0 / x ** 2 for x in h))
-    w = material.param.vals.reshape(-1, len(elements))
+    par = material.param
+    w = par.vals.reshape(-1, len(elements)) if hasattr(par, "vals") else np.asarray(par.w).reshape(1, -1)
     rho = w / (w / material.rho_pure).sum(axis=1, keepdims=True)
     sp = np.array([StoppingPower(e)(E0) for e in elements])
     smin = float((rho @ sp).min())
-    return max(8, int(math.ceil((E0 - E1) * LAMBDA_MAX[N] * inv / (cfl * smin))))
+    return (E0 - E1) * LAMBDA_MAX[N] * inv / smin
+
+
+def cfl_number(N, gspec, material, elements, E0, E1, n_steps):
+    """nu = dE lambda_max sqrt(sum 1/h^2) / S_min of an energy march: the explicit
+    stability number that the reference prints but never enforces
+    (pn_solver/pn_intensities.jl:2-8).  S_min is the smallest rho . sp over the
+    material at the highest energy.  The march is stable for nu below about 1;
+    the parity tests and workloads here run at nu <= 0.5."""
+    return _cfl_x(N, gspec, material, elements, E0, E1) / n_steps
+
+
+def cfl_steps(N, gspec, material, elements, E0, E1, cfl=0.5, floor=8):
+    """Smallest step count (at least `floor`) with nu <= cfl (see cfl_number)."""
+    return max(floor, int(math.ceil(_cfl_x(N, gspec, material, elements, E0, E1) / cfl)))
 
 
 def draw_spec(rng: np.random.Generator, sizes_1d=(100, 400), sizes_2d=(64, 192), max_steps=None) -> Spec:
@@ -126,10 +141,17 @@ class _Prepared:
         self.W = np.ascontiguousarray(si.W)
         self.I_bar = np.ascontiguousarray(spec.I_bar, dtype=np.float64)
         self.I_out = np.zeros(len(p.x_rays))
+        self.rho_bar, self.gen = [None] * 8, None
+        if want_rho_bar:
+            self.want_gradients()
+
+    def want_gradients(self):
+        """Host arrays that receive the request's density gradients and
+        generation sums at completion (before the request is submitted)."""
+        p = self.spec.problem
         ne = len(p.elements)
-        self.rho_bar = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)] if want_rho_bar \
-            else [None] * 8
-        self.gen = np.zeros((int(np.prod(p.gspec.class_size(0))), len(p.x_rays))) if want_rho_bar else None
+        self.rho_bar = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)]
+        self.gen = np.zeros((int(np.prod(p.gspec.class_size(0))), len(p.x_rays)))
 
 
 def prepare(spec: Spec, want_rho_bar: bool = False) -> _Prepared:
@@ -173,9 +195,16 @@ class Server:
         return t.value
 
     def wait(self, ticket: int) -> Timing:
+        """Blocks until the request is complete and releases its ticket."""
         tm = Timing()
         check(lib().pnx_server_wait(self.handle, ticket, C.byref(tm)))
+        self._live.pop(ticket, None)
         return tm
+
+    def cancel(self, ticket: int):
+        """Cancels a pending or admitted request; its outputs are not written and
+        its Timing.status becomes PNX_ERR_CANCELLED."""
+        check(lib().pnx_server_cancel(self.handle, ticket))
 
     def poll(self, ticket: int):
         d, tm = C.c_int32(), Timing()

@@ -73,8 +73,10 @@ def make_desc(problem: ForwardPNProblem, ckpt_every: int = 0) -> Desc:
 class DeviceProblem:
     """A batch of beams resident on the device (pnx_problem)."""
 
-    def __init__(self, problem: ForwardPNProblem, inputs: SolveInputs, ctx: Context | None = None,
+    def __init__(self, problem: ForwardPNProblem, inputs: SolveInputs | None, ctx: Context | None = None,
                  ckpt_every: int = 0, adjoint: bool = True):
+        """inputs None: the material is set later on the device (set_cell_fractions,
+        then set_absorption)."""
         self.problem = problem
         self.ctx = ctx or default_context()
         self.desc = make_desc(problem, ckpt_every)
@@ -87,7 +89,9 @@ class DeviceProblem:
             check(lib().pnx_problem_set_records(h, dptr(fwd), dptr(adj), dptr(dens)))
         else:
             check(lib().pnx_problem_set_records(h, dptr(fwd), None, None))
-        self.set_material(inputs)
+        self.inputs, self.cells, self.has_absorption = None, None, False
+        if inputs is not None:
+            self.set_material(inputs)
         mms = problem.beam_moments()
         m8 = (_lib._dp * 8)(*[dptr(m) for m in mms])
         profs = [problem.beam_profiles(b) for b in range(problem.n_beams)]
@@ -188,6 +192,10 @@ class DeviceProblem:
         out = np.zeros((self.problem.n_beams, len(self.problem.x_rays)))
         check(lib().pnx_forward(self.handle, dptr(out)))
         return out
+
+    def reserve_adjoint(self, exact: bool = False):
+        """Allocate the adjoint sweep's device buffers now (not in the first forward_adjoint)."""
+        check(lib().pnx_problem_reserve_adjoint(self.handle, int(exact)))
 
     def forward_adjoint(self, I_bar, exact: bool = False, want_rho_bar: bool = True):
         """Returns (I, rho_bar[8] per class (P_c, n_e) summed over beams, gen (P_eee, n_lines)).

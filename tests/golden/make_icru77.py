@@ -4,10 +4,11 @@ Source data: ICRU Report 77 (Elastic Scattering of Electrons and Positrons,
 2007), as tabulated in the reference's pn_solver/stopping_power.jl:51-154
 (energies `E_sp` in MeV, `data_tot_sp[Z]` in MeV/cm).  Only the numbers are
 taken (they are physical data); this script reads that file as text and writes
-`icru77_tot_sp.json` next to itself.  It is run once, in a checkout that has the
-reference mounted, and its output is committed.
+`pnepma_amd/data/icru77_tot_sp.json`.  It is run once, in a checkout that has
+the reference mounted, and its output is committed: it is the provenance of
+that table, not part of the package.
 
-    python pnepma_amd/data/make_icru77.py /root/reference/pn_solver/stopping_power.jl
+    python tests/golden/make_icru77.py /root/reference/pn_solver/stopping_power.jl
 """
 import json
 import os
@@ -37,7 +38,8 @@ def main(path):
         "energy_MeV": energies,
         "tot_sp_MeV_per_cm": rows,
     }
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "icru77_tot_sp.json")
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    dst = os.path.join(root, "pnepma_amd", "data", "icru77_tot_sp.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=0)
     print("wrote", dst, len(energies), "energies,", len(rows), "elements")

@@ -33,15 +33,48 @@ def interior_mask(gspec, c):
     return (X & Y & Z).ravel()
 
 
-def small_1d(N=5, cells=40, n_steps=24, seed=0, E0=15e3, E1=5e3, depth=0.4e-4):
-    p, mat, stds, det = workloads.slab1d(N=N, cells=cells, n_steps=n_steps, depth=depth, E0=E0, E1=E1, seed=seed)
+def stable_steps(p, mat, cfl=0.5) -> int:
+    """Energy steps that put the march at the stability number nu <= cfl."""
+    from pnepma_amd import serving
+    return serving.cfl_steps(p.N, p.gspec, mat, p.elements, p.E_initial, p.E_cutoff, cfl=cfl)
+
+
+def cfl_of(p, mat) -> float:
+    from pnepma_amd import serving
+    return serving.cfl_number(p.N, p.gspec, mat, p.elements, p.E_initial, p.E_cutoff, p.n_steps)
+
+
+def small_1d(N=5, cells=40, n_steps=None, seed=0, E0=15e3, E1=5e3, depth=0.4e-4, unstable_ok=False):
+    """A small Cu/Ni slab.  n_steps None: the step count of nu <= 0.5.  An
+    explicit step count must also be stable unless unstable_ok (single-step
+    arithmetic tests, which compare one step of a random state)."""
+    kw = dict(N=N, cells=cells, depth=depth, E0=E0, E1=E1, seed=seed)
+    p, mat, stds, det = workloads.slab1d(n_steps=n_steps or 8, **kw)
+    if n_steps is None:
+        p, mat, stds, det = workloads.slab1d(n_steps=stable_steps(p, mat), **kw)
+    elif not unstable_ok:
+        assert cfl_of(p, mat) <= 0.5, f"unstable test configuration: nu = {cfl_of(p, mat):.2f}"
     return p, mat, stds, det
 
 
-def small_2d(N=3, M=(12, 10), n_steps=16, n_beams=2, seed=1, E0=15e3, E1=6e3, extent=(0.3e-4, 0.2e-4)):
-    p, mat, stds, det = workloads.raster2d(N=N, M=M, n_steps=n_steps, extent=extent, E0=E0, E1=E1,
-                                           n_beams=n_beams, seed=seed)
+def small_2d(N=3, M=(12, 10), n_steps=None, n_beams=2, seed=1, E0=15e3, E1=6e3, extent=(0.3e-4, 0.2e-4),
+             unstable_ok=False):
+    """A small Fe/Cr/Ni raster problem; n_steps as small_1d."""
+    kw = dict(N=N, M=M, extent=extent, E0=E0, E1=E1, n_beams=n_beams, seed=seed)
+    p, mat, stds, det = workloads.raster2d(n_steps=n_steps or 8, **kw)
+    if n_steps is None:
+        p, mat, stds, det = workloads.raster2d(n_steps=stable_steps(p, mat), **kw)
+    elif not unstable_ok:
+        assert cfl_of(p, mat) <= 0.5, f"unstable test configuration: nu = {cfl_of(p, mat):.2f}"
     return p, mat, stds, det
+
+
+def assert_physical(I):
+    """Intensities of a stable march: finite, nonzero and of one sign (the
+    reference's beam moments are -int Y vMF, so they come out negative)."""
+    I = np.asarray(I)
+    assert np.all(np.isfinite(I)), I
+    assert np.all(I < 0) or np.all(I > 0), I
 
 
 def random_state(gspec, N, rng, active_only=True):

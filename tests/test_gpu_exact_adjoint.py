@@ -20,9 +20,9 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("ckpt", [0, 3])
 def test_exact_adjoint_vs_dense(oracle_lib, case, ckpt):
     if case == "1d":
-        p, mat, _, det = small_1d(N=5, cells=16, n_steps=8, E0=15e3, E1=10e3)
+        p, mat, _, det = small_1d(N=5, cells=16, E0=15e3, E1=10e3)
     else:
-        p, mat, _, det = small_2d(N=3, M=(8, 6), n_steps=6, n_beams=1, E1=12e3)
+        p, mat, _, det = small_2d(N=3, M=(8, 6), n_beams=1, E1=12e3)
     si = inputs(p, mat, det)
     ib = np.linspace(1.0, -0.5, len(p.x_rays))
     I, rb, gen = numpy_step.discrete_pullback(p, si, oracle_lib.advection_dense, oracle_lib.boundary_dense, ib)
@@ -38,7 +38,7 @@ def test_exact_adjoint_vs_dense(oracle_lib, case, ckpt):
 
 
 def test_exact_adjoint_vs_central_differences(oracle_lib):
-    p, mat, _, det = small_2d(N=3, M=(10, 8), n_steps=10, n_beams=1, E1=10e3)
+    p, mat, _, det = small_2d(N=3, M=(10, 8), n_beams=1, E1=10e3)
     si = inputs(p, mat, det)
     ib = np.array([1.0, -0.6, 0.4])
     dp = DeviceProblem(p, si)

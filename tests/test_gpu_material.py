@@ -17,6 +17,7 @@ from pnepma_amd import solver, workloads
 from pnepma_amd.grid import GridSpec
 from pnepma_amd.material import CellFractions, MassFractionMaterial
 from pnepma_amd.physics import XRay, element
+from pnepma_amd import serving
 from pnepma_amd.problem import ForwardPNProblem, solve_inputs
 from pnepma_amd.solver import DeviceProblem, cell_aligned
 
@@ -25,7 +26,8 @@ from .helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _problem(M, N=3, n_steps=30, seed=4, n_beams=2):
+def _problem(M, N=3, n_steps=None, seed=4, n_beams=2):
+    """n_steps None: the stable step count (nu <= 0.5, at least 30)."""
     rng = np.random.default_rng(seed)
     names = ["Fe", "Cr", "Ni"]
     els = [element(s) for s in names]
@@ -36,11 +38,13 @@ def _problem(M, N=3, n_steps=30, seed=4, n_beams=2):
     p = ForwardPNProblem(N=N, gspec=g, elements=els, beam_energy=(15e3, workloads.SIGMA_E),
                          beam_positions=[(0.0, float(y), 0.0) for y in ys], beam_sigma=50 * workloads.NM,
                          beam_direction=((-1.0, 0.0, 0.0), workloads.KAPPA), E_initial=15.5e3, E_cutoff=5e3,
-                         n_steps=n_steps, source_mode="lateral")
+                         n_steps=n_steps or 30, source_mode="lateral")
     p.x_rays = [XRay(s) for s in names]
     vals = workloads.inclusion_fractions(tuple(M), len(els), 4, rng)
     mat = MassFractionMaterial(els, CellFractions(vals, *[workloads.cell_edges(a) for a in g.axes],
                                                   embedding=vals.reshape(-1, len(els))[0]))
+    if n_steps is None:
+        p.n_steps = max(30, serving.cfl_steps(N, g, mat, els, p.E_initial, p.E_cutoff))
     return p, mat
 
 

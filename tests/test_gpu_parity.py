@@ -2,6 +2,11 @@
 
 Tolerances: single steps 1e-12 (relative to the largest value), k-ratios and
 intensities 1e-10, gradients (density and generation pullbacks) 1e-8.
+
+Every whole march here runs at a stable step count (nu <= 0.5, see
+serving.cfl_number), so the compared intensities are the physical solution
+(finite, of one sign).  The single-step tests are arithmetic identities: one
+step of a random state, where the stability of a march does not enter.
 """
 import numpy as np
 import pytest
@@ -9,7 +14,7 @@ import pytest
 from pnepma_amd import grid as G
 from pnepma_amd.solver import DeviceProblem
 
-from .helpers import inputs, interior_mask, random_state, rel_err, small_1d, small_2d
+from .helpers import assert_physical, cfl_of, inputs, interior_mask, random_state, rel_err, small_1d, small_2d
 
 pytestmark = pytest.mark.gpu
 
@@ -68,51 +73,32 @@ def test_step_forward_3d(oracle_lib):
     _step_case(p, mat, det, oracle_lib, adjoint=False, record=3)
 
 
-@pytest.mark.parametrize("ckpt", [0, 4])
-def test_forward_adjoint_3d(oracle_lib, ckpt):
-    """3D P5 forward + reference adjoint of one beam against the oracle: I and rho_bar of all eight classes."""
-    from pnepma_amd.grid import GridSpec
-    from pnepma_amd.material import ConstFractions, MassFractionMaterial
-    p, mat, _, det = small_2d(N=5, M=(8, 6), n_steps=10)
-    p.gspec = GridSpec.make((9, 7, 5), (-0.3e-4, 0.0), (-1e-5, 1e-5), (-0.8e-5, 0.8e-5))
-    p.beam_positions = [(0.0, 0.0, 0.0)]
-    mat = MassFractionMaterial(mat.elements, ConstFractions([0.3, 0.3, 0.4]))
-    si = inputs(p, mat, det)
-    dp = DeviceProblem(p, si, ckpt_every=ckpt)
-    ib = np.array([[1.0, -0.5, 0.25]])
-    I, rb, gen = dp.forward_adjoint(ib)
-    I_ref, rb_ref, _, _ = oracle_lib.forward_adjoint(p, si, ib[0], beam=0)
-    assert rel_err(I[0], I_ref) < 1e-10
-    for c in p.gspec.active_classes():
-        e = rel_err(rb[c], rb_ref[c])
-        print(f"rho_bar {G.CLASS_NAMES[c]} rel err {e:.3e}", flush=True)
-        assert e < 1e-8
-
-
 def test_forward_intensities_1d(oracle_lib):
-    p, mat, stds, det = small_1d(N=5, cells=60, n_steps=60)
+    p, mat, stds, det = small_1d(N=5, cells=60)
     si = inputs(p, mat, det)
     dp = DeviceProblem(p, si, adjoint=False)
     I = dp.forward()[0]
     I_ref = oracle_lib.forward(p, si)
-    print("I gpu", I, "ref", I_ref, flush=True)
+    print(f"{p.n_steps} steps, nu {cfl_of(p, mat):.3f}: I gpu", I, "ref", I_ref, flush=True)
+    assert_physical(I_ref)
     assert rel_err(I, I_ref) < 1e-10
 
 
 def test_forward_intensities_2d_beams(oracle_lib):
-    p, mat, stds, det = small_2d(N=5, M=(16, 12), n_steps=24, n_beams=3)
+    p, mat, stds, det = small_2d(N=5, M=(16, 12), n_beams=3)
     si = inputs(p, mat, det)
     dp = DeviceProblem(p, si, adjoint=False)
     I = dp.forward()
     for b in range(p.n_beams):
         I_ref = oracle_lib.forward(p, si, beam=b)
-        print("beam", b, I[b], I_ref, flush=True)
+        print(f"{p.n_steps} steps, nu {cfl_of(p, mat):.3f}: beam", b, I[b], I_ref, flush=True)
+        assert_physical(I_ref)
         assert rel_err(I[b], I_ref) < 1e-10
 
 
 @pytest.mark.parametrize("ckpt", [0, 5])
 def test_forward_adjoint_2d(oracle_lib, ckpt):
-    p, mat, stds, det = small_2d(N=3, M=(10, 8), n_steps=14, n_beams=2)
+    p, mat, stds, det = small_2d(N=3, M=(10, 8), n_beams=2)
     si = inputs(p, mat, det)
     dp = DeviceProblem(p, si, ckpt_every=ckpt)
     ib = np.array([[1.0, -0.5, 0.25], [0.3, 0.7, -1.1]])
@@ -121,6 +107,7 @@ def test_forward_adjoint_2d(oracle_lib, ckpt):
     acb_ref = np.zeros_like(si.ac)
     for b in range(p.n_beams):
         I_ref, rbr, acb, noab = oracle_lib.forward_adjoint(p, si, ib[b], beam=b)
+        assert_physical(I_ref)
         assert rel_err(I[b], I_ref) < 1e-10
         for c in range(8):
             rb_ref[c] += rbr[c]

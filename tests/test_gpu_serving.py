@@ -109,7 +109,7 @@ def _spec_3d(N, M, n_steps, seed):
 @pytest.mark.parametrize("tape", [True, False])
 def test_served_3d_equals_alone(tape):
     """3D requests (the grid lanes' DIM = 3 kernels) mixed with 2D ones."""
-    specs = [_spec_3d(3, (10, 8, 6), 24, 1), _spec_3d(5, (8, 6, 6), 20, 2), _spec_3d(3, (12, 6, 5), 18, 3)]
+    specs = [_spec_3d(3, (10, 8, 6), 26, 1), _spec_3d(5, (8, 6, 6), 23, 2), _spec_3d(3, (12, 6, 5), 24, 3)]
     specs += _specs(n=4, seed=13)
     preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
     srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=64)
@@ -128,3 +128,37 @@ def test_served_3d_equals_alone(tape):
         for c in p.gspec.active_classes():
             assert rel_err(pr.rho_bar[c], rb[c]) < 1e-12
         assert rel_err(pr.gen, gen) < 1e-12
+
+
+def test_cancel_pending_and_admitted():
+    """A cancelled request completes with status PNX_ERR_CANCELLED and writes no
+    output; the requests served beside it are unaffected."""
+    import time
+    from pnepma_amd._lib import PNX_ERR_CANCELLED
+    specs = _specs(n=6, seed=17)
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=64)
+    t_pending = srv.submit(preps[0], arrival_s=srv.clock() + 3600.0)  # still pending when cancelled
+    tickets = [srv.submit(pr) for pr in preps[1:]]
+    t_end = time.perf_counter() + 10.0
+    while time.perf_counter() < t_end:  # cancel request 1 once it is admitted
+        done, tm = srv.poll(tickets[0])
+        if done or tm.admit_s > 0:
+            break
+    srv.cancel(tickets[0])
+    srv.cancel(t_pending)
+    tm = srv.wait(t_pending)
+    assert tm.status == PNX_ERR_CANCELLED
+    assert not np.any(preps[0].I_out)
+    tm = srv.wait(tickets[0])
+    print("request 1 after cancel: status", tm.status, "ticks", tm.ticks, flush=True)
+    assert tm.status in (0, PNX_ERR_CANCELLED)
+    if tm.status == PNX_ERR_CANCELLED:
+        assert not np.any(preps[1].I_out)
+    for t in tickets[1:]:
+        assert srv.wait(t).status == 0
+    srv.drain()
+    srv.close()
+    for spec, pr in zip(specs[2:], preps[2:]):
+        I, rb, gen = _alone(spec)
+        assert rel_err(pr.I_out, I) < 1e-12

@@ -126,7 +126,7 @@ def test_boundary_blocks_halfrange(N, oracle_lib):
 def test_uniform_state_decays_exactly(oracle_lib):
     """Far from the boundaries a spatially uniform state only decays:
     u' = u exp(-C dE / S) per moment over one full Strang step."""
-    p, mat, _, det = small_1d(N=5, cells=30, n_steps=10)
+    p, mat, _, det = small_1d(N=5, cells=30)
     mat = MassFractionMaterial(mat.elements, ConstFractions([0.4, 0.6]))
     si = inputs(p, mat, det)
     cm = G.class_moments(p.N)
@@ -178,18 +178,18 @@ def _dense_vs_oracle(p, mat, det, oracle_lib, adjoint, record, seed):
 
 @pytest.mark.parametrize("N", [1, 3, 5])
 def test_step_dense_numpy_1d(N, oracle_lib):
-    p, mat, _, det = small_1d(N=N, cells=12, n_steps=8)
+    p, mat, _, det = small_1d(N=N, cells=12)
     _dense_vs_oracle(p, mat, det, oracle_lib, False, 2, N)
 
 
 @pytest.mark.parametrize("N,adjoint", [(3, False), (5, False), (3, True), (7, True)])
 def test_step_dense_numpy_2d(N, adjoint, oracle_lib):
-    p, mat, _, det = small_2d(N=N, M=(6, 5), n_steps=6)
+    p, mat, _, det = small_2d(N=N, M=(6, 5))
     _dense_vs_oracle(p, mat, det, oracle_lib, adjoint, 3, 10 + N)
 
 
 def test_pullback_dense_numpy(oracle_lib):
-    p, mat, _, det = small_2d(N=3, M=(6, 5), n_steps=4, n_beams=1)
+    p, mat, _, det = small_2d(N=3, M=(6, 5), n_beams=1, E1=13e3)
     si = inputs(p, mat, det)
     ib = np.array([1.0, -0.4, 0.7])
     I, rb, acb, noab = numpy_step.pullback(p, si, oracle_lib.advection_dense, oracle_lib.boundary_dense, ib)
@@ -204,7 +204,7 @@ def test_pullback_dense_numpy(oracle_lib):
 # ------------------------------------------------------------ whole solve
 def test_source_linearity_exact(oracle_lib):
     """Intensities are linear in the beam source: scaling it by 2^-20 scales them exactly."""
-    p, mat, stds, det = small_1d(N=3, cells=30, n_steps=20)
+    p, mat, stds, det = small_1d(N=3, cells=30)
     si = inputs(p, mat, det)
     I1 = oracle_lib.forward(p, si)
     p.beam_sigma_scale = None
