@@ -134,6 +134,23 @@ int pnx_problem_set_beams(pnx_problem* p, const double* const mms[8], const doub
 /* calc_intensities for every beam: I_out[b*n_lines + r]. */
 int pnx_forward(pnx_problem* p, double* I_out);
 
+/* pnx_forward that allocates nothing: no hipMalloc, hipFree or graph, only
+ * kernel launches, one device-to-host copy of I_out and a stream
+ * synchronisation.  No counterpart in the reference; it is for callers that
+ * cannot allocate device memory while they run (code under HIP graph capture
+ * of their own, tight serving loops, harnesses that forbid allocation).  The
+ * intensity partial sums go to a fixed device scratch array of 64 MiB per
+ * device, held until the process ends.  A problem needs (n_steps + 1) *
+ * n_beams * n_tiles * 4 * n_lines + n_beams * n_lines doubles of it, n_tiles
+ * being the number of 256-cell tiles covering the grid (256 x 1 in 1D, 32 x 8
+ * per z-plane in 2D and 3D): the 2D P7 128x96 raster with 32 beams, 3 lines
+ * and 400 steps needs 59 MB.  A larger problem returns PNX_ERR_ARG without
+ * launching anything.  The scratch needs no clearing: every
+ * value is written before it is read.  Calls on one device share that array,
+ * so calls that overlap on one device (different threads or streams) race;
+ * serialise them.  Results equal pnx_forward's. */
+int pnx_forward_noalloc(pnx_problem* p, double* I_out);
+
 /* Forward pass, then the reference adjoint sweep with checkpointing every
  * ckpt_every steps and forward recomputation between checkpoints.
  *   I_bar[b*n_lines + r]   seed of the pullback per beam

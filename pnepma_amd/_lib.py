@@ -52,6 +52,7 @@ SIGNATURES = {
     "pnx_problem_material_gradient": (C.c_int, [C.c_void_p, _dp]),
     "pnx_march_sequence": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
     "pnx_forward": (C.c_int, [C.c_void_p, _dp]),
+    "pnx_forward_noalloc": (C.c_int, [C.c_void_p, _dp]),
     "pnx_problem_reserve_adjoint": (C.c_int, [C.c_void_p, C.c_int]),
     "pnx_forward_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),
     "pnx_forward_adjoint_exact": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(_dp), _dp]),

@@ -11,10 +11,19 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "pnx_host.hpp"
+
+// Device scratch of the allocation-free forward (pnx_forward_noalloc): the
+// intensity partial sums of every beam and energy step, then the reduced
+// intensities.  Each device the library runs on holds its own copy until the
+// process ends.  Host code never uses this name as an address: it uses the
+// per-device pointer from hipGetSymbolAddress (pnx_context.scratch).
+constexpr size_t kScratchDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace pnx {
 double advection_coefficient(int l, int k, int lf, int kf, int d);
@@ -34,6 +43,26 @@ using namespace pnx;
 namespace {
 
 int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of the current device's `scratch`, resolved once per
+// device (the device must be current).
+int resolve_scratch(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return fail(PNX_ERR_ARG, "device index out of range");
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PNX_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[device];
+  return PNX_OK;
+}
 
 #define HIP_TRY(expr)                                                                               \
   do {                                                                                              \
@@ -108,8 +137,11 @@ int pnx_context_create(int device, pnx_context** out) {
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(PNX_ERR_ARG, "device index out of range");
   HIP_TRY(hipSetDevice(device));
+  double* scr = nullptr;
+  if (int rc = resolve_scratch(device, &scr)) return rc;
   auto* c = new pnx_context;
   c->device = device;
+  c->scratch = scr;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(PNX_ERR_DEVICE, "hipStreamCreate failed");
@@ -519,19 +551,23 @@ int adjoint_step(pnx_problem* p, int i, double* lam) {
   return PNX_OK;
 }
 
-int enqueue_forward(pnx_problem* p) {
+// The forward march of every beam; the intensity partial sums go to
+// `partial` (partial_beam doubles per beam) and the intensities to I_dev.
+int enqueue_forward(pnx_problem* p, double* partial, double* I_dev) {
   hipStream_t s = p->stream;
   HIP_TRY(launch_fill(p->state.p, (long)p->L.beam_stride * p->d.n_beams, 0.0, s));
   double* st = p->state.p;
   int rc = forward_run(
       p, 1, p->d.n_steps, st, [&](int) { return st; },
-      [&](int k) { return p->partial.p + (long)k * p->g.partial_step; });
+      [&](int k) { return partial + (long)k * p->g.partial_step; });
   if (rc) return rc;
-  HIP_TRY(launch_reduce_intensity(p->partial.p, p->g.partial_step, p->g.partial_beam, p->L.n_tiles * kWavesPerBlock, p->rec_f.p, p->g.rec_len,
-                                  p->g.sigma_off, p->d.n_steps, p->d.n_lines, p->g.dE, p->L.dvol, p->d.n_beams,
-                                  p->I_dev.p, s));
+  HIP_TRY(launch_reduce_intensity(partial, p->g.partial_step, p->g.partial_beam, p->L.n_tiles * kWavesPerBlock,
+                                  p->rec_f.p, p->g.rec_len, p->g.sigma_off, p->d.n_steps, p->d.n_lines, p->g.dE,
+                                  p->L.dvol, p->d.n_beams, I_dev, s));
   return PNX_OK;
 }
+
+int enqueue_forward(pnx_problem* p) { return enqueue_forward(p, p->partial.p, p->I_dev.p); }
 
 int ensure_graph(pnx_problem* p) {
   if (p->fwd_graph) return PNX_OK;
@@ -806,6 +842,28 @@ static int forward_adjoint_impl(pnx_problem* p, bool exact, const double* I_bar,
     HIP_TRY(hipStreamSynchronize(s));
     unpack_lines(p->g, h.data(), gen_out);
   }
+  HIP_TRY(hipStreamSynchronize(s));
+  return PNX_OK;
+}
+
+int pnx_forward_noalloc(pnx_problem* p, double* I_out) {
+  int rc = check_ready(p, false);
+  if (rc) return rc;
+  if (!I_out) return fail(PNX_ERR_ARG, "null output");
+  double* sp = p->ctx->scratch;
+  if (!sp) return fail(PNX_ERR_STATE, "the context has no scratch address");
+  // partial sums [beam][step][wavefront][line], then the intensities 256-byte aligned
+  const size_t n_part = (size_t)p->g.partial_beam * p->d.n_beams;
+  const size_t off_I = (n_part + 31) / 32 * 32;
+  const size_t need = off_I + (size_t)p->d.n_beams * p->d.n_lines;
+  if (need > kScratchDoubles)
+    return fail(PNX_ERR_ARG, "the allocation-free forward needs " + std::to_string(need * sizeof(double)) +
+                                 " bytes of scratch for this problem; its maximum is " +
+                                 std::to_string(kScratchDoubles * sizeof(double)) +
+                                 " bytes (fewer beams or energy steps, or use pnx_forward)");
+  if ((rc = enqueue_forward(p, sp, sp + off_I))) return rc;
+  hipStream_t s = p->stream;
+  HIP_TRY(hipMemcpyAsync(I_out, sp + off_I, sizeof(double) * p->d.n_beams * p->d.n_lines, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;
 }

@@ -13,6 +13,7 @@
 struct pnx_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  double* scratch = nullptr;  // this device's copy of the allocation-free forward's scratch
 };
 
 namespace pnx {
