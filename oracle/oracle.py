@@ -68,6 +68,8 @@ def lib():
         L.ref_class_moments.argtypes = [C.c_int, C.c_int]
         L.ref_class_points.argtypes = [C.POINTER(Desc), C.c_int]
         L.ref_record_len.argtypes = [C.POINTER(Desc)]
+        L.ref_set_plant.argtypes = [C.c_int]
+        L.ref_set_plant.restype = None
         _lib = L
     return _lib
 
@@ -158,6 +160,11 @@ def step(problem, si, u, rec, adjoint=False, q_eee=None, beam: int = 0):
     rc = lib().ref_step(C.byref(d), int(adjoint), _ptr(np.ascontiguousarray(rec)), rho, m8, p8, _ptr(q), u8)
     if rc:
         raise RuntimeError("ref_step failed")
+
+
+def set_plant(k: int):
+    """Plants a known mistake in the oracle's adjoint (see pnx_ref.h); 0 removes it."""
+    lib().ref_set_plant(int(k))
 
 
 def advection_dense(N, c, d, nc, nf):

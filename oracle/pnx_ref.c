@@ -275,6 +275,11 @@ static void csc_build(csc_t* A, const cls_t* to, const cls_t* from, int d, doubl
   A->colptr[from->n] = nz;
 }
 
+/* Planted mistakes, for the tests that show the oracle's pins detect them
+ * (0 = the faithful restatement; see ref_set_plant in pnx_ref.h). */
+static int g_plant = 0;
+void ref_set_plant(int k) { g_plant = k; }
+
 static int prob_init(prob_t* P, const ref_desc* d, int adjoint, const ref_inputs* in) {
   memset(P, 0, sizeof(*P));
   if (d->N < 0 || d->N > 9 || d->n_e < 1 || d->n_e > 8) return -1;
@@ -298,6 +303,8 @@ static int prob_init(prob_t* P, const ref_desc* d, int adjoint, const ref_inputs
       if (odd_in(c, a)) {
         P->B[c][a] = malloc(sizeof(double) * nm * P->cls[sw(c, a)].n + 1);
         ref_boundary_dense(d->N, c, a, adjoint, P->B[c][a]);
+        if (adjoint && g_plant == 1)
+          for (int q = 0; q < nm * P->cls[sw(c, a)].n; ++q) P->B[c][a][q] = -P->B[c][a][q];
       }
     }
     if (!adjoint && odd_in(c, 0)) {
@@ -595,7 +602,7 @@ static void integrate_adjoint_densities(prob_t* F, const ref_inputs* in, const d
           size_t p = pidx(F, c, i, j, k);
           for (int e = 0; e < ne; ++e)
             for (int m = 0; m < nm; ++m) {
-              double ce = cl[e * (N + 1) + F->cls[c].l[m]];
+              double ce = g_plant == 2 ? 0.0 : cl[e * (N + 1) + F->cls[c].l[m]];
               rho_bar[c][p * ne + e] +=
                   lam[c][p * nm + m] * (s[e] * du[c][p * nm + m] + ce * u[c][p * nm + m]) * F->dvol * dE * int_fac;
             }
@@ -624,7 +631,7 @@ static void adjoint_source(prob_t* A, const ref_inputs* in, const double* sigma,
     double q = 0.0;
     for (int r = 0; r < nl; ++r)
       q -= sigma[r] * in->ac[(size_t)p * nl + r] * in->noa[(size_t)p * ne + in->el_idx[r]] * I_bar[r];
-    A->Q[0][(size_t)p * n0] = q;
+    A->Q[0][(size_t)p * n0] = g_plant == 3 ? -q : q;
   }
 }
 
