@@ -28,9 +28,12 @@ class Inputs(C.Structure):
                 ("mms", _dp * 8), ("ppos", _dp * 8)]
 
 
+SOURCES = ("pnx_ref.c", "pnx_ref_material.c")
+
+
 def build(force: bool = False) -> str:
-    src = os.path.join(HERE, "pnx_ref.c")
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+    newest = max(os.path.getmtime(os.path.join(HERE, f)) for f in SOURCES + ("pnx_ref.h",))
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
         subprocess.check_call(["make", "-s", "-C", HERE])
     return LIB
 
@@ -46,8 +49,8 @@ def use_native(out_dir: str) -> str:
     benchmark's CPU baseline."""
     global _lib, _path
     out = os.path.join(out_dir, "libpnx_oracle_native.so")
-    subprocess.check_call(["gcc", "-O3", "-march=native", "-std=c99", "-fPIC", "-shared", "-o", out,
-                           os.path.join(HERE, "pnx_ref.c"), "-lm"])
+    subprocess.check_call(["gcc", "-O3", "-march=native", "-std=c99", "-fPIC", "-shared", "-o", out] +
+                          [os.path.join(HERE, f) for f in SOURCES] + ["-lm"])
     _lib, _path = None, out
     lib()
     return out
@@ -178,3 +181,146 @@ def boundary_dense(N, c, d, adjoint, nc, nf):
     if lib().ref_boundary_dense(N, c, d, int(adjoint), _ptr(out)):
         raise ValueError("class not odd in axis")
     return out
+
+
+# ------------------------------------------------------------------ material
+class RefBoxes(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("edges", _dp * 3),
+                ("n_e", C.c_int), ("vals", _dp), ("embedding", _dp), ("rho_pure", _dp)]
+
+
+def _bind_material(L):
+    B = C.POINTER(RefBoxes)
+    L.ref_box_index.argtypes = [B, _dp]
+    L.ref_box_index.restype = C.c_long
+    L.ref_box_index_floor.argtypes = [B, _dp]
+    L.ref_box_index_floor.restype = C.c_long
+    L.ref_component_densities.argtypes = [B, _dp, C.c_long, _dp]
+    L.ref_component_densities_pullback.argtypes = [B, _dp, C.c_long, _dp, _dp, _dp]
+    L.ref_absorption.argtypes = [B, _dp, C.c_long, _dp, C.c_int, _dp, C.c_double, _dp]
+    L.ref_absorption_pullback.argtypes = [B, _dp, C.c_long, _dp, C.c_int, _dp, C.c_double, _dp, _dp, _dp, _dp]
+    L.ref_number_of_atoms.argtypes = [B, _dp, C.c_long, _dp, _dp]
+    L.ref_number_of_atoms_pullback.argtypes = [B, _dp, C.c_long, _dp, _dp, _dp, _dp]
+
+
+class Boxes:
+    """The oracle's view of a MassFraction material: a box parametrization with
+    uniform slices (CellFractions), or the embedding everywhere (ConstFractions)."""
+
+    def __init__(self, material):
+        par = material.param
+        self.ne = len(material.elements)
+        self.s = RefBoxes()
+        if hasattr(par, "vals"):
+            self.shape = par.vals.shape[:3]
+            self.vals = np.ascontiguousarray(par.vals.transpose(2, 1, 0, 3).reshape(-1, self.ne))
+            self.emb = np.ascontiguousarray(par.embedding, dtype=np.float64)
+            self.edges = [np.ascontiguousarray(e, dtype=np.float64) for e in par.edges]
+            for a in range(3):
+                self.s.n[a] = self.shape[a]
+                self.s.lo[a] = float(par.edges[a][0])
+                self.s.hi[a] = float(par.edges[a][-1])
+                self.s.edges[a] = _ptr(self.edges[a])
+        else:
+            self.shape = (0, 0, 0)
+            self.vals = np.zeros((1, self.ne))
+            self.emb = np.ascontiguousarray(par.w, dtype=np.float64)
+        self.rho_pure = np.ascontiguousarray(material.rho_pure, dtype=np.float64)
+        self.s.n_e = self.ne
+        self.s.vals = _ptr(self.vals)
+        self.s.embedding = _ptr(self.emb)
+        self.s.rho_pure = _ptr(self.rho_pure)
+
+    def ref(self):
+        L = lib()
+        if not hasattr(L, "_material_bound"):
+            _bind_material(L)
+            L._material_bound = True
+        return C.byref(self.s)
+
+    def params_grad(self, vals_bar):
+        """[box x fastest][element] -> the CellFractions.params order (x slowest)."""
+        nx, ny, nz = self.shape
+        return vals_bar.reshape(nz, ny, nx, self.ne).transpose(2, 1, 0, 3).ravel()
+
+
+def _pts(pts):
+    return np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+
+
+def component_densities(material, pts):
+    b, pts = Boxes(material), _pts(pts)
+    out = np.zeros((pts.shape[0], b.ne))
+    lib().ref_component_densities(b.ref(), _ptr(pts), pts.shape[0], _ptr(out))
+    return out
+
+
+def box_index(material, x, literal_floor: bool = False):
+    """Box of point x (-1: embedding); literal_floor: the reference's floating-point floor."""
+    b = Boxes(material)
+    ref = b.ref()
+    f = lib().ref_box_index_floor if literal_floor else lib().ref_box_index
+    return int(f(ref, _ptr(np.ascontiguousarray(x, dtype=np.float64))))
+
+
+def component_densities_pullback(material, pts, rho_bar):
+    """(gradient in CellFractions.params order, embedding gradient)."""
+    b, pts = Boxes(material), _pts(pts)
+    vb, eb = np.zeros_like(b.vals), np.zeros(b.ne)
+    lib().ref_component_densities_pullback(b.ref(), _ptr(pts), pts.shape[0],
+                                           _ptr(np.ascontiguousarray(rho_bar, dtype=np.float64)), _ptr(vb), _ptr(eb))
+    return b.params_grad(vb), eb
+
+
+def absorption(material, pts, mac, detector, x_b0):
+    """ac[p][r] from every point; mac[r][e]."""
+    b, pts = Boxes(material), _pts(pts)
+    mac = np.ascontiguousarray(mac, dtype=np.float64)
+    out = np.zeros((pts.shape[0], mac.shape[0]))
+    lib().ref_absorption(b.ref(), _ptr(pts), pts.shape[0], _ptr(mac), mac.shape[0],
+                         _ptr(np.ascontiguousarray(detector, dtype=np.float64)), float(x_b0), _ptr(out))
+    return out
+
+
+def absorption_pullback(material, pts, mac, detector, x_b0, ac, ac_bar):
+    b, pts = Boxes(material), _pts(pts)
+    mac = np.ascontiguousarray(mac, dtype=np.float64)
+    vb, eb = np.zeros_like(b.vals), np.zeros(b.ne)
+    lib().ref_absorption_pullback(b.ref(), _ptr(pts), pts.shape[0], _ptr(mac), mac.shape[0],
+                                  _ptr(np.ascontiguousarray(detector, dtype=np.float64)), float(x_b0),
+                                  _ptr(np.ascontiguousarray(ac)), _ptr(np.ascontiguousarray(ac_bar)), _ptr(vb),
+                                  _ptr(eb))
+    return b.params_grad(vb), eb
+
+
+def number_of_atoms(material, pts, atom_mass):
+    b, pts = Boxes(material), _pts(pts)
+    out = np.zeros((pts.shape[0], b.ne))
+    lib().ref_number_of_atoms(b.ref(), _ptr(pts), pts.shape[0], _ptr(np.ascontiguousarray(atom_mass)), _ptr(out))
+    return out
+
+
+def number_of_atoms_pullback(material, pts, atom_mass, noa_bar):
+    b, pts = Boxes(material), _pts(pts)
+    vb, eb = np.zeros_like(b.vals), np.zeros(b.ne)
+    lib().ref_number_of_atoms_pullback(b.ref(), _ptr(pts), pts.shape[0], _ptr(np.ascontiguousarray(atom_mass)),
+                                       _ptr(np.ascontiguousarray(noa_bar, dtype=np.float64)), _ptr(vb), _ptr(eb))
+    return b.params_grad(vb), eb
+
+
+def material_gradient(problem, material, detector, mac, rho_bar, ac_bar, noa_bar):
+    """The material pullbacks of the reference adjoint (pn_intensities.jl:406-410)
+    from the oracle's rho_bar (per class), ac_bar and noa_bar: the densities of
+    every class, the absorption factors and the numbers of atoms.  Returns the
+    gradient in CellFractions.params order."""
+    g = problem.gspec
+    am = np.array([e.atom_mass_g for e in material.elements])
+    eee = g.class_points(0)
+    grad = np.zeros(material.param.n_params)
+    for c in range(8):
+        if rho_bar[c].size and np.any(rho_bar[c]):
+            grad += component_densities_pullback(material, g.class_points(c), rho_bar[c])[0]
+    ac = absorption(material, eee, mac, detector, problem.beam_positions[0][0])
+    grad += absorption_pullback(material, eee, mac, detector, problem.beam_positions[0][0], ac, ac_bar)[0]
+    grad += number_of_atoms_pullback(material, eee, am, noa_bar)[0]
+    return grad

@@ -100,6 +100,39 @@ int ref_forward_adjoint(const ref_desc* d, const ref_inputs* in, const double* f
 int ref_step(const ref_desc* d, int adjoint, const double* rec, const double* const rho[8], const double* const mms[8],
              const double* const ppos[8], const double* q_eee, double* const u[8]);
 
+/* ------------------------------------------------------------- material
+ * MassFraction material over a box parametrization whose slices are
+ * range(lo[a], hi[a], length = n[a] + 1), with edges[a] its n[a] + 1 edge
+ * coordinates (pnx_ref_material.c).  Points are [point][3]; densities,
+ * gradients [point][element]; vals [box][element] with boxes counted x
+ * fastest.  n[a] = 0 on every axis: the embedding everywhere. */
+typedef struct {
+  int n[3];
+  double lo[3], hi[3];
+  const double* edges[3];
+  int n_e;
+  const double* vals;
+  const double* embedding;
+  const double* rho_pure;
+} ref_boxes;
+
+/* The box of a point (-1: the embedding) in exact arithmetic, as the oracle's
+ * densities use it, and as the reference's floor((x - lo) / step) computes it
+ * in floating point; they differ only for points on interior edges. */
+long ref_box_index(const ref_boxes* B, const double* x);
+long ref_box_index_floor(const ref_boxes* B, const double* x);
+int ref_component_densities(const ref_boxes* B, const double* pts, long n_pts, double* rho);
+int ref_component_densities_pullback(const ref_boxes* B, const double* pts, long n_pts, const double* rho_bar,
+                                     double* vals_bar, double* emb_bar);
+int ref_absorption(const ref_boxes* B, const double* pts, long n_pts, const double* mac, int n_lines,
+                   const double* x_d, double x_b0, double* ac);
+int ref_absorption_pullback(const ref_boxes* B, const double* pts, long n_pts, const double* mac, int n_lines,
+                            const double* x_d, double x_b0, const double* ac, const double* ac_bar, double* vals_bar,
+                            double* emb_bar);
+int ref_number_of_atoms(const ref_boxes* B, const double* pts, long n_pts, const double* atom_mass, double* noa);
+int ref_number_of_atoms_pullback(const ref_boxes* B, const double* pts, long n_pts, const double* atom_mass,
+                                 const double* noa_bar, double* vals_bar, double* emb_bar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ restatement (pnepma_amd.material, pn_material.jl / pn_parametrization_boxed.jl).
 import numpy as np
 import pytest
 
-from pnepma_amd import solver, workloads
+from pnepma_amd import physics, solver, workloads
 from pnepma_amd.grid import GridSpec
 from pnepma_amd.material import CellFractions, MassFractionMaterial
 from pnepma_amd.physics import XRay, element
@@ -130,3 +130,47 @@ def test_device_material_gradient_matches_host(M):
     e = rel_err(g_dev, g_host)
     print(f"whole material gradient device vs host rel err {e:.3e}", flush=True)
     assert e < 1e-11
+
+
+def _oracle_inputs(p, mat, det):
+    """SolveInputs built from the oracle's material layer alone."""
+    from oracle import oracle
+    from pnepma_amd.problem import SolveInputs, el_index
+    g = p.gspec
+    mac = np.array([[physics.mass_absorption(x, e) for e in mat.elements] for x in p.x_rays])
+    am = np.array([e.atom_mass_g for e in mat.elements])
+    rho = [oracle.component_densities(mat, g.class_points(c)) for c in range(8)]
+    noa = oracle.number_of_atoms(mat, g.class_points(0), am)
+    ac = oracle.absorption(mat, g.class_points(0), mac, det, p.beam_positions[0][0])
+    ei = el_index(mat.elements, p.x_rays)
+    return SolveInputs(rho, noa, ac, ei, np.ascontiguousarray(ac * noa[:, ei])), mac
+
+
+@pytest.mark.parametrize("M", [(40, 1, 1), (20, 14, 1), (10, 8, 6)], ids=["1d", "2d", "3d"])
+def test_device_material_layer_matches_oracle(M, oracle_lib):
+    """Densities, absorption factors and the whole material gradient of the
+    reference adjoint computed on the device (pnx_problem_set_cell_fractions,
+    pnx_problem_set_absorption, pnx_problem_material_gradient) against the
+    oracle's restatement of pn_material.jl:236-256, the box parametrization
+    and x_ray_intensities.jl:43-153, composed with the oracle's reference
+    adjoint: intensities 1e-10, absorption factors 1e-13, gradient 1e-8."""
+    p, mat = _problem(M, n_beams=2)
+    det = workloads.detector_40deg()
+    si_o, mac = _oracle_inputs(p, mat, det)
+    ib = np.random.default_rng(11).uniform(0.5, 1.5, (p.n_beams, len(p.x_rays)))
+    dp = DeviceProblem(p, None, ckpt_every=8)
+    dp.set_cell_fractions(mat, None)
+    ac = dp.set_absorption(mat, det)
+    e_ac = rel_err(ac, si_o.ac)
+    I, _, _ = dp.forward_adjoint(ib, want_rho_bar=False)
+    g_dev = dp.material_gradient()
+    dp.close()
+    g_ref = np.zeros_like(g_dev)
+    for b in range(p.n_beams):
+        I_ref, rb, acb, noab = oracle_lib.forward_adjoint(p, si_o, ib[b], beam=b)
+        assert rel_err(I[b], I_ref) < 1e-10
+        g_ref += oracle_lib.material_gradient(p, mat, det, mac, rb, acb, noab)
+    e_g = rel_err(g_dev, g_ref)
+    print(f"{M}: absorption factors {e_ac:.3e}, material gradient {e_g:.3e}", flush=True)
+    assert e_ac < 1e-13
+    assert e_g < 1e-8

@@ -92,3 +92,49 @@ def test_raster2d_batch_properties():
     q = _single_beam(p, 7)
     Iq = DeviceProblem(q, solve_inputs(q, mat, det), adjoint=False).forward()[0]
     assert rel_err(Iq, I16[7]) < 1e-13
+
+
+def test_raster2d_misfit_gradient_20_cells():
+    """The 2D gradient workload: 128x96 P7, 32 beams, 400 energy steps,
+    checkpoints every 16, least-squares k-ratio misfit against targets
+    k_truth (1 + N(0, 0.01)) at truth + N(0, 0.05) (seed 2).  The gradient with
+    respect to the per-cell mass fractions (exact adjoint of the discrete
+    march, material pullbacks on the device) against central differences
+    (h = 1e-6) of the misfit on 20 random cells the beams reach, to 1e-5.  The reference
+    adjoint's gradient is printed beside it: it discretises the continuous
+    adjoint, so it differs from the differences by its first-order gap."""
+    from pnepma_amd.reconstruct import KRatioMisfit, standard_intensities
+    p, truth, stds, det = workloads.raster2d(n_steps=400)
+    rng = np.random.default_rng(2)
+    I_std = standard_intensities(p, stds, det)
+    mf = KRatioMisfit(p, truth, det, I_std, np.zeros((p.n_beams, len(p.x_rays))), ckpt_every=16)
+    k_truth = mf.kratios(truth.param.params())
+    mf.k_exp = k_truth * (1 + rng.normal(0, 0.01, k_truth.shape))
+    ne = len(truth.elements)
+    x = np.clip(truth.param.params() + rng.normal(0, 0.05, truth.param.n_params), 0.01, None).reshape(-1, ne)
+    x = (x / x.sum(axis=1, keepdims=True)).ravel()
+    L, g = mf.loss_and_grad(x)
+    mf_ref = KRatioMisfit(p, truth, det, I_std, mf.k_exp, ckpt_every=16, adjoint="reference")
+    _, g_ref = mf_ref.loss_and_grad(x)
+    mf_ref.close()
+    # 20 random cells among those the beams reach (a gradient above 1 % of the
+    # largest): deeper cells have gradients below 1e-20, which central
+    # differences of a misfit of 1e-3 cannot resolve
+    gc = np.abs(g).reshape(-1, ne).max(axis=1)
+    cells = rng.choice(np.flatnonzero(gc > 1e-2 * gc.max()), 20, replace=False)
+    h = 1e-6
+    worst = worst_ref = 0.0
+    for cell in cells:
+        i = cell * ne + int(rng.integers(ne))
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (mf.loss(xp) - mf.loss(xm)) / (2 * h)
+        scale = max(abs(fd), 1e-2 * np.max(np.abs(g)))
+        e, e_ref = abs(g[i] - fd) / scale, abs(g_ref[i] - fd) / scale
+        print(f"cell {cell} param {i}: exact {g[i]:.6e} reference {g_ref[i]:.6e} fd {fd:.6e} "
+              f"rel {e:.2e} / {e_ref:.2e}", flush=True)
+        worst, worst_ref = max(worst, e), max(worst_ref, e_ref)
+    mf.close()
+    print(f"L = {L:.6e}; worst rel err: exact adjoint {worst:.2e}, reference adjoint {worst_ref:.2e}", flush=True)
+    assert worst < 1e-5
