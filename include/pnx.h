@@ -78,6 +78,54 @@ int64_t pnx_problem_device_bytes(const pnx_problem* p);
  * adj and dens may be NULL for forward-only use. */
 int pnx_problem_set_records(pnx_problem* p, const double* fwd, const double* adj, const double* dens);
 
+/* Physical tables of a problem's elements and lines, from which the library
+ * builds the energy records (pnx_problem_records).  Per element e: the
+ * stopping power sp[e] (eV cm^2/g) at the knots sp_E (eV, ascending; the
+ * reference's ICRU-77 table divided by the density, stopping_power.jl:37-44)
+ * and the transport coefficients tc[e] ([n_tc][tc_stride] cm^2/g, degree l
+ * in column l, l = 0..N) at the energies tc_E (eV, ascending), interpolated
+ * linearly in log E and clamped to the table.  Per line r: line_edge[r], the
+ * K edge (eV) of its element.  The beam energy distribution is
+ * Normal(beam_E_mean, beam_E_sd). */
+typedef struct {
+  int32_t n_knots;
+  const double* sp_E;
+  const double* sp[8];
+  int32_t n_tc;
+  int32_t tc_stride;
+  const double* tc_E;
+  const double* tc[8];
+  double line_edge[8];
+  double beam_E_mean;
+  double beam_E_sd;
+} pnx_physics;
+
+/* The energy records of pnx_problem_set_records built on the host in C++
+ * (update_coefficients!, pn_solver/pn_problem.jl:157-190;
+ * update_boundary_condition!, :205-211; the energies of
+ * pn_solver/pn_intensities.jl:10-30, 341-401): fwd, adj and dens are
+ * (n_steps + 1) * record length doubles each; adj and dens may be NULL
+ * (forward only).  The emission cross section is the library's K-shell
+ * stand-in 6.51e-14 * 2 * 0.9 ln(U) / (U E_K^2), U = E / E_K (the
+ * reference's NeXLCore Bote2009 table is not available).  Fails with
+ * PNX_ERR_ARG if an energy lies outside the stopping-power table. */
+int pnx_problem_records(const pnx_desc* desc, const pnx_physics* physics, double* fwd, double* adj, double* dens);
+
+/* Beam moments of a von Mises-Fisher beam direction (beam_moments,
+ * pn_solver/pn_matrices.jl:331-338) in the Funk-Hecke closed form
+ * -Y_lk(direction) i_l(kappa) / i_0(kappa): mms[c] receives the n_c values of
+ * every x-odd class c (NULL for the others). */
+int pnx_beam_moments(int N, const double* direction, double kappa, double* const mms[8]);
+
+/* Beam position profiles of one beam (setup_boundary_condition!,
+ * pn_solver/pn_problem.jl:192-203): ppos[c] receives the isotropic normal
+ * pdf (standard deviation sigma) on the (y, z) points of x-odd class c, y
+ * fastest (classes whose ppos[c] is NULL are skipped).  literal != 0 evaluates it at x = the first x point,
+ * as the reference does (the factor exp(-(x_min - x_b)^2 / (2 sigma^2))
+ * underflows for deep slabs); literal = 0 drops that global factor. */
+int pnx_beam_profiles(const pnx_desc* desc, const double* beam_position, double sigma, int literal,
+                      double* const ppos[8]);
+
 /* Material inputs.  rho[c] (c = 0..7, eee..ooo) is [point][element] over the
  * reference's class-c points; entries for classes the grid never updates may be
  * NULL.  W is [eee point][line] = absorption factor * number of atoms of the

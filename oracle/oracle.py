@@ -28,7 +28,7 @@ class Inputs(C.Structure):
                 ("mms", _dp * 8), ("ppos", _dp * 8)]
 
 
-SOURCES = ("pnx_ref.c", "pnx_ref_material.c")
+SOURCES = ("pnx_ref.c", "pnx_ref_material.c", "pnx_ref_records.c")
 
 
 def build(force: bool = False) -> str:
@@ -324,3 +324,57 @@ def material_gradient(problem, material, detector, mac, rho_bar, ac_bar, noa_bar
     grad += absorption_pullback(material, eee, mac, detector, problem.beam_positions[0][0], ac, ac_bar)[0]
     grad += number_of_atoms_pullback(material, eee, am, noa_bar)[0]
     return grad
+
+
+# ------------------------------------------------------------------- records
+class RefTables(C.Structure):
+    _fields_ = [("n_knots", C.c_int), ("sp_E", _dp), ("sp", _dp * 8), ("n_tc", C.c_int), ("tc_stride", C.c_int),
+                ("tc_E", _dp), ("tc", _dp * 8), ("line_edge", C.c_double * 8), ("beam_E_mean", C.c_double),
+                ("beam_E_sd", C.c_double)]
+
+
+def _tables(problem):
+    """The oracle's element tables from the same data the package uses: the
+    ICRU-77 rows over the density and the transport-coefficient tabulation."""
+    from pnepma_amd import physics
+    T = RefTables()
+    keep = []
+    for e, el in enumerate(problem.elements):
+        kn, sp = physics.stopping_power_table(el)
+        tE, tc = physics.transport_table(el)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (kn, sp, tE, tc)]
+        keep += arrs
+        T.sp_E, T.sp[e], T.tc_E, T.tc[e] = [_ptr(a) for a in arrs]
+        T.n_knots, T.n_tc, T.tc_stride = arrs[0].size, arrs[3].shape[0], arrs[3].shape[1]
+    for r, x in enumerate(problem.x_rays):
+        T.line_edge[r] = x.edge
+    T.beam_E_mean, T.beam_E_sd = problem.beam_energy
+    return T, keep
+
+
+def _bind_records(L):
+    L.ref_records.argtypes = [C.POINTER(Desc), C.POINTER(RefTables), _dp, _dp, _dp]
+    L.ref_record_at.argtypes = [C.POINTER(Desc), C.POINTER(RefTables), C.c_double, C.c_double, C.c_int, _dp]
+
+
+def records(problem):
+    """(fwd, adj, dens) as the solver takes them, from the oracle."""
+    L = lib()
+    _bind_records(L)
+    T, keep = _tables(problem)
+    d = desc_of(problem)
+    out = tuple(np.zeros((problem.n_steps + 1, L.ref_record_len(C.byref(d)))) for _ in range(3))
+    if L.ref_records(C.byref(d), C.byref(T), *[_ptr(a) for a in out]):
+        raise ValueError("an energy lies outside the stopping-power table")
+    return out
+
+
+def record_at(problem, Ec, Es, adjoint=False):
+    L = lib()
+    _bind_records(L)
+    T, keep = _tables(problem)
+    d = desc_of(problem)
+    out = np.zeros(L.ref_record_len(C.byref(d)))
+    if L.ref_record_at(C.byref(d), C.byref(T), float(Ec), float(Es), int(adjoint), _ptr(out)):
+        raise ValueError("energy outside the stopping-power table")
+    return out

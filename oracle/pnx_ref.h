@@ -133,6 +133,27 @@ int ref_number_of_atoms(const ref_boxes* B, const double* pts, long n_pts, const
 int ref_number_of_atoms_pullback(const ref_boxes* B, const double* pts, long n_pts, const double* atom_mass,
                                  const double* noa_bar, double* vals_bar, double* emb_bar);
 
+/* -------------------------------------------------------------- records
+ * Per-energy coefficients (pnx_ref_records.c) from the element tables: the
+ * stopping power sp[e] at the knots sp_E, the transport coefficients tc[e]
+ * ([n_tc][tc_stride], degree l in column l) at the energies tc_E, the K edge
+ * of each line, the beam energy distribution. */
+typedef struct {
+  int n_knots;
+  const double* sp_E;
+  const double* sp[8];
+  int n_tc, tc_stride;
+  const double* tc_E;
+  const double* tc[8];
+  double line_edge[8];
+  double beam_E_mean, beam_E_sd;
+} ref_tables;
+
+/* fwd, adj, dens: (n_steps + 1) records each, as the solver takes them. */
+int ref_records(const ref_desc* d, const ref_tables* T, double* fwd, double* adj, double* dens);
+/* One record with coefficients (and beam bin) at Ec and cross sections at Es. */
+int ref_record_at(const ref_desc* d, const ref_tables* T, double Ec, double Es, int adjoint, double* rec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,12 @@ class Request(C.Structure):
                 ("rho_bar", _dp * 8), ("gen_out", _dp)]
 
 
+class Physics(C.Structure):
+    _fields_ = [("n_knots", C.c_int32), ("sp_E", _dp), ("sp", _dp * 8), ("n_tc", C.c_int32), ("tc_stride", C.c_int32),
+                ("tc_E", _dp), ("tc", _dp * 8), ("line_edge", C.c_double * 8), ("beam_E_mean", C.c_double),
+                ("beam_E_sd", C.c_double)]
+
+
 class Timing(C.Structure):
     _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
                 ("ticks", C.c_int32), ("taped", C.c_int32)]
@@ -43,6 +49,9 @@ SIGNATURES = {
     "pnx_problem_destroy": (C.c_int, [C.c_void_p]),
     "pnx_problem_device_bytes": (C.c_int64, [C.c_void_p]),
     "pnx_problem_set_records": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "pnx_problem_records": (C.c_int, [C.POINTER(Desc), C.POINTER(Physics), _dp, _dp, _dp]),
+    "pnx_beam_moments": (C.c_int, [C.c_int, _dp, C.c_double, _dp * 8]),
+    "pnx_beam_profiles": (C.c_int, [C.POINTER(Desc), _dp, C.c_double, C.c_int, _dp * 8]),
     "pnx_problem_set_material": (C.c_int, [C.c_void_p, _dp * 8, _dp]),
     "pnx_problem_set_beams": (C.c_int, [C.c_void_p, _dp * 8, C.POINTER(_dp)]),
     "pnx_problem_set_cell_fractions": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),

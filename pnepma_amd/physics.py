@@ -160,6 +160,19 @@ def transport_coefficients(el: Element, E, N: int) -> np.ndarray:
     return (1.0 - t) * tab[i, :N + 1] + t * tab[i + 1, :N + 1]
 
 
+def transport_table(el: Element) -> tuple:
+    """(energies, tc[energy][l] for l = 0..9): the tabulation that
+    transport_coefficients interpolates, as the C++ record builder takes it."""
+    transport_coefficients(el, 1000.0, 9)
+    return _TC_GRID, _TC_CACHE[el.symbol]
+
+
+def stopping_power_table(el: Element) -> tuple:
+    """(knot energies in eV, sp in eV cm^2 / g): the interpolation table of StoppingPower."""
+    sp = StoppingPower(el)
+    return sp.knots, sp.values
+
+
 def _transport_coefficients_direct(el: Element, E, N: int) -> np.ndarray:
     E = np.atleast_1d(np.asarray(E, dtype=np.float64))
     eta = 3.4 * el.Z ** (2.0 / 3.0) / E

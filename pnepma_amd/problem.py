@@ -121,10 +121,52 @@ class ForwardPNProblem:
         n = len(Ecoef)
         return np.concatenate([s.reshape(n, -1), c.reshape(n, -1), g.reshape(n, 1), sig.reshape(n, -1)], axis=1)
 
+    def physics_tables(self):
+        """The pnx_physics tables of this problem's elements and lines (and the
+        arrays they point into)."""
+        from ._lib import Physics, dptr
+        ph = Physics()
+        knots = None
+        keep = []
+        for e, el in enumerate(self.elements):
+            kn, sp = physics.stopping_power_table(el)
+            tE, tc = physics.transport_table(el)
+            knots = np.ascontiguousarray(kn, dtype=np.float64) if knots is None else knots
+            sp = np.ascontiguousarray(sp, dtype=np.float64)
+            tc = np.ascontiguousarray(tc, dtype=np.float64)
+            keep += [sp, tc]
+            ph.sp[e] = dptr(sp)
+            ph.tc[e] = dptr(tc)
+            ph.n_tc = tc.shape[0]
+            ph.tc_stride = tc.shape[1]
+            tE = np.ascontiguousarray(tE, dtype=np.float64)
+            keep.append(tE)
+            ph.tc_E = dptr(tE)
+        keep.append(knots)
+        ph.n_knots = knots.size
+        ph.sp_E = dptr(knots)
+        for r, x in enumerate(self.x_rays):
+            ph.line_edge[r] = x.edge
+        ph.beam_E_mean, ph.beam_E_sd = self.beam_energy
+        return ph, keep
+
     def records(self):
-        """(fwd, adj, dens) record tables, each (n_steps + 1, record_len), C-contiguous."""
+        """(fwd, adj, dens) record tables, each (n_steps + 1, record_len), C-contiguous,
+        built by the library's C++ set-up (pnx_problem_records)."""
         if "records" in self._cache:
             return self._cache["records"]
+        import ctypes as C
+        from ._lib import check, dptr, lib
+        from .solver import make_desc
+        ph, keep = self.physics_tables()
+        out = tuple(np.zeros((self.n_steps + 1, self.record_len)) for _ in range(3))
+        check(lib().pnx_problem_records(C.byref(make_desc(self)), C.byref(ph), *[dptr(a) for a in out]))
+        self._cache["records"] = out
+        return out
+
+    def records_host(self):
+        """The same record tables computed in numpy (kept as the package's
+        cross-check of the C++ set-up)."""
         fe = self.forward_energies()
         Ec = np.array([self.E_initial] + [a for a, _ in fe])
         Es = np.array([self.E_initial] + [b for _, b in fe])
@@ -132,18 +174,41 @@ class ForwardPNProblem:
         Er = np.array(self.reverse_energies())
         adj = self._records(Er + self.dE / 2, Er + self.dE / 2, adjoint=True)
         dens = self._records(Er, Er, adjoint=False)
-        out = tuple(np.ascontiguousarray(a) for a in (fwd, adj, dens))
-        self._cache["records"] = out
-        return out
+        return tuple(np.ascontiguousarray(a) for a in (fwd, adj, dens))
 
     # ----------------------------------------------------------------- beam
     def beam_moments(self):
-        """mms per x-odd class (pn_solver/pn_problem.jl:192-203), None for x-even classes."""
+        """mms per x-odd class (pn_solver/pn_problem.jl:192-203), None for x-even
+        classes, from the library's C++ set-up (pnx_beam_moments)."""
+        import ctypes as C
+        from ._lib import _dp, check, dptr, lib
+        cm = class_moments(self.N)
+        direction, kappa = self.beam_direction
+        out = [np.zeros(len(cm[c])) if odd_in(c, 0) else None for c in range(8)]
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        check(lib().pnx_beam_moments(self.N, dptr(d), float(kappa), (_dp * 8)(*[dptr(x) for x in out])))
+        return out
+
+    def beam_moments_host(self):
+        """The same moments from the numpy Funk-Hecke closed form."""
         cm = class_moments(self.N)
         direction, kappa = self.beam_direction
         return [physics.vmf_beam_moments(cm[c], direction, kappa) if odd_in(c, 0) else None for c in range(8)]
 
     def beam_profiles(self, beam: int):
+        """Position pdf on the (y, z) points of each x-odd class (y fastest), from
+        the library's C++ set-up (pnx_beam_profiles); see beam_profiles_host."""
+        import ctypes as C
+        from ._lib import _dp, check, dptr, lib
+        from .solver import make_desc
+        g = self.gspec
+        out = [np.zeros(g.class_size(c)[1] * g.class_size(c)[2]) if odd_in(c, 0) else None for c in range(8)]
+        xb = np.ascontiguousarray(self.beam_positions[beam], dtype=np.float64)
+        check(lib().pnx_beam_profiles(C.byref(make_desc(self)), dptr(xb), float(self.beam_sigma),
+                                      int(self.source_mode == "literal"), (_dp * 8)(*[dptr(x) for x in out])))
+        return out
+
+    def beam_profiles_host(self, beam: int):
         """Position pdf on the (y, z) points of each x-odd class (y fastest).
 
         The reference evaluates the 3D isotropic normal at x = x_min, the first
