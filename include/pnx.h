@@ -282,6 +282,40 @@ typedef struct {
   double* gen_out;               /* optional [eee point][line], written at completion */
 } pnx_request;
 
+/* A served request given as what the reference's calc_intensities takes
+ * (pn_solver/pn_intensities.jl:325-335): the problem's geometry and energy
+ * grid, its element tables, a MassFraction material with one box per grid
+ * cell (boxes centred on the even points, edges at the odd points) and the
+ * detector.  At submit the library builds the records, beam moments and beam
+ * profiles (as pnx_problem_records, pnx_beam_moments, pnx_beam_profiles); at
+ * admission the densities, absorption factors and line weights are computed
+ * on the device (as pnx_problem_set_cell_fractions and
+ * pnx_problem_set_absorption); at completion, if grad_out is given, the whole
+ * material gradient (as pnx_problem_material_gradient).  Inputs are copied at
+ * submit. */
+typedef struct {
+  pnx_desc desc;                 /* n_beams must be 1; ckpt_every 0 = about sqrt(n_steps) */
+  const pnx_physics* physics;
+  const double* cell_w;          /* [cell][element] mass fractions, cells x fastest (Mx*My*Mz) */
+  const double* rho_pure;        /* [element] pure-element densities (g/cm^3) */
+  const double* embedding;       /* [element] fractions outside the boxes (for the absorption rays) */
+  const double* mac;             /* [line][element] mass absorption coefficients (cm^2/g) */
+  const double* atom_mass;       /* [element] atomic masses (g) */
+  const int32_t* line_element;   /* [line] element index of each line */
+  double detector[3];            /* detector position (cm) */
+  double beam_position[3];       /* beam centre; x = the surface, where the absorption rays are cut */
+  double beam_sigma;             /* beam profile standard deviation (cm) */
+  double beam_direction[3];      /* von Mises-Fisher mean direction */
+  double beam_kappa;             /* and concentration */
+  int32_t literal_source;        /* 1: the reference's beam pdf at x_min (see pnx_beam_profiles) */
+  const double* I_bar;           /* n_lines pullback seed */
+  double arrival_s;              /* admission not before this server-clock time */
+  double* I_out;                 /* n_lines, written at completion */
+  double* rho_bar[8];            /* optional [point][element] per class */
+  double* gen_out;               /* optional [eee point][line] */
+  double* grad_out;              /* optional [cell][element]: d(I_bar . I)/d(cell fractions) */
+} pnx_request_spec;
+
 typedef struct {
   double arrival_s;   /* server clock, seconds */
   double admit_s;
@@ -308,6 +342,9 @@ double pnx_server_clock(const pnx_server* s);
  * pullback always keeps the whole tape (pn_solver/pn_intensities.jl:353-404). */
 int pnx_server_set_tape_limit(pnx_server* s, int64_t bytes);
 int pnx_server_submit(pnx_server* s, const pnx_request* req, int64_t* ticket);
+/* Submits a request given as cell fractions (pnx_request_spec): its
+ * host preparation is the library's C++ set-up in the caller's thread. */
+int pnx_server_submit_spec(pnx_server* s, const pnx_request_spec* req, int64_t* ticket);
 int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t);
 /* Blocks until the request is complete; this releases the ticket (a later
  * poll, wait or cancel of it returns PNX_ERR_ARG). */

@@ -33,6 +33,15 @@ class Physics(C.Structure):
                 ("beam_E_sd", C.c_double)]
 
 
+class RequestSpec(C.Structure):
+    _fields_ = [("desc", Desc), ("physics", C.POINTER(Physics)), ("cell_w", _dp), ("rho_pure", _dp),
+                ("embedding", _dp), ("mac", _dp), ("atom_mass", _dp), ("line_element", C.POINTER(C.c_int32)),
+                ("detector", C.c_double * 3), ("beam_position", C.c_double * 3), ("beam_sigma", C.c_double),
+                ("beam_direction", C.c_double * 3), ("beam_kappa", C.c_double), ("literal_source", C.c_int32),
+                ("I_bar", _dp), ("arrival_s", C.c_double), ("I_out", _dp), ("rho_bar", _dp * 8), ("gen_out", _dp),
+                ("grad_out", _dp)]
+
+
 class Timing(C.Structure):
     _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("status", C.c_int32),
                 ("ticks", C.c_int32), ("taped", C.c_int32)]
@@ -75,6 +84,7 @@ SIGNATURES = {
     "pnx_server_clock": (C.c_double, [C.c_void_p]),
     "pnx_server_set_tape_limit": (C.c_int, [C.c_void_p, C.c_int64]),
     "pnx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "pnx_server_submit_spec": (C.c_int, [C.c_void_p, C.POINTER(RequestSpec), C.POINTER(C.c_int64)]),
     "pnx_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.POINTER(Timing)]),
     "pnx_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Timing)]),
     "pnx_server_cancel": (C.c_int, [C.c_void_p, C.c_int64]),

@@ -6,6 +6,16 @@ namespace pnx {
 
 static bool order_supported(int N) { return N == 1 || N == 3 || N == 5 || N == 7 || N == 9; }
 
+double axis_point(const pnx_desc& d, int a, int odd, int i) {
+  const int M = d.M[a];
+  if (M == 1) return d.lo[a];
+  const double h = (d.hi[a] - d.lo[a]) / (M - 1);
+  const double lo = odd ? d.lo[a] - h / 2 : d.lo[a], hi = odd ? d.hi[a] + h / 2 : d.hi[a];
+  const int n = odd ? M + 1 : M;
+  if (i == n - 1) return hi;
+  return i * ((hi - lo) / (n - 1)) + lo;
+}
+
 int make_geometry(const pnx_desc& d, Geometry& g, std::string& err) {
   if (!order_supported(d.N)) {
     err = "PN order must be one of 1, 3, 5, 7, 9";

@@ -41,6 +41,11 @@ struct Geometry {
 // Validates desc and fills g; returns PNX_OK or an error code with message in err.
 int make_geometry(const pnx_desc& d, Geometry& g, std::string& err);
 
+// Coordinate of point i of axis a (odd: the M + 1 points from lo - h/2 to
+// hi + h/2; even: the M points from lo to hi), computed as numpy.linspace
+// does; a singleton axis has the one point lo.
+double axis_point(const pnx_desc& d, int a, int odd, int i);
+
 // Reference AoS [point][vec] of class c -> padded device layout [vec][V] at dst.
 void pack_class(const Geometry& g, int c, const double* src, int vec, double* dst);
 void unpack_class(const Geometry& g, int c, const double* src, int vec, double* dst);

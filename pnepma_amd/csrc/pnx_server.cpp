@@ -213,10 +213,31 @@ class PinnedPool {
 // doubles each device buffer of a request needs
 struct Sizes {
   long state, ckpt, seg, tape, rho, W, rec, prof, partial, rb, gen, small;
+  // requests given as cell fractions (pnx_server_submit_spec): the fractions,
+  // the absorption inputs, the absorption table and factors, the gradient
+  long cell, abs, mu, ac, grad, extra;
 };
 
-Sizes request_sizes(const Geometry& g, const pnx_desc& d) {
+// Doubles of the absorption inputs of a request given as cell fractions:
+// even points and box edges per axis, mac, atomic masses, embedding,
+// pure-element densities, and the line elements (eight int32 in four doubles).
+long abs_inputs(const Geometry& g, const pnx_desc& d) {
+  long n = 0;
+  for (int a = 0; a < 3; ++a) n += 2L * d.M[a] + 1;
+  return n + (long)d.n_lines * d.n_elements + 3L * d.n_elements + 4;
+}
+
+Sizes request_sizes(const Geometry& g, const pnx_desc& d, bool spec = false) {
   Sizes s{};
+  if (spec) {
+    const long n_cells = (long)d.M[0] * d.M[1] * d.M[2];
+    s.cell = n_cells * d.n_elements;
+    s.abs = abs_inputs(g, d);
+    s.mu = (n_cells + 1) * d.n_lines;
+    s.ac = (long)d.n_lines * g.L.V;
+    s.grad = n_cells * d.n_elements;
+    s.extra = (long)d.n_elements * g.L.V;
+  }
   s.state = g.L.beam_stride;
   s.ckpt = (long)g.nck * s.state;
   s.seg = (long)g.K * s.state;
@@ -248,9 +269,9 @@ Sizes taped_sizes(const Sizes& z, int n) {
 // not cleared: its slots are written before they are read, and the kernels
 // read its ghost slots only to replace them by rebuilt boundary values.
 struct Offsets {
-  long rec_f, rec_a, rec_d, rho, W, prof, mms, ibar, in_end;
-  long state, lam, rb, gen, seg, zero_end;
-  long ckpt, tape, partial, I, total;
+  long rec_f, rec_a, rec_d, rho, W, prof, mms, ibar, cell, abs, in_end;
+  long state, lam, rb, gen, seg, ac, extra, zero_end;
+  long ckpt, tape, partial, I, mu, grad, total;
 };
 
 Offsets request_offsets(const Sizes& z) {
@@ -269,17 +290,23 @@ Offsets request_offsets(const Sizes& z) {
   o.prof = take(z.prof);
   o.mms = take(8 * 16);
   o.ibar = take(8);
+  o.cell = take(z.cell);
+  o.abs = take(z.abs);
   o.in_end = p;
   o.state = take(z.state);
   o.lam = take(z.state);
   o.rb = take(z.rb);
   o.gen = take(z.gen);
   o.seg = take(z.seg);
+  o.ac = take(z.ac);
+  o.extra = take(z.extra);
   o.zero_end = p;
   o.ckpt = take(z.ckpt);
   o.tape = take(z.tape);
   o.partial = take(z.partial);
   o.I = take(8);
+  o.mu = take(z.mu);
+  o.grad = take(z.grad);
   o.total = p;
   return o;
 }
@@ -303,12 +330,19 @@ struct Request {
   double* I_out = nullptr;
   double* rho_bar[8] = {};
   double* gen_out = nullptr;
+  double* grad_out = nullptr;    // spec requests: the whole material gradient [cell][element]
   double* pinned_out = nullptr;  // results land here (inside `pinned`), then go to the caller's arrays
+  // given as cell fractions: the material and absorption are set on the device at admission
+  bool spec = false;
+  long abs_pts[3] = {}, abs_edges[3] = {}, abs_mac = 0, abs_am = 0, abs_emb = 0, abs_rp = 0, abs_el = 0;
+  double inv_span[3] = {}, detector[3] = {}, beam_x0 = 0.0;
+  AbsorptionGeometry ag{};
   // device
   size_t off = 0, bytes = 0;
   double *state = nullptr, *lam = nullptr, *ckpt = nullptr, *seg = nullptr, *rho = nullptr, *W = nullptr,
          *rec_f = nullptr, *rec_a = nullptr, *rec_d = nullptr, *prof = nullptr, *partial = nullptr, *rb = nullptr,
-         *gen = nullptr, *mms = nullptr, *I_dev = nullptr, *ibar = nullptr;
+         *gen = nullptr, *mms = nullptr, *I_dev = nullptr, *ibar = nullptr, *cell_w = nullptr, *abs = nullptr,
+         *mu = nullptr, *ac = nullptr, *grad = nullptr, *extra = nullptr;
   // schedule
   MarchPos pos = march_start();
   int ops_left = 0;            // operations not yet issued
@@ -422,6 +456,33 @@ int admit(pnx_server* s, Request& r) {
   // state, lam, rho_bar, gen start at zero; the segment states are written
   // before they are read, but their ghost slots are kept finite
   if (launch_fill(r.state, o.zero_end - o.state, 0.0, st) != hipSuccess) return -1;
+  if (r.spec) {
+    // the densities of every class and the absorption factors and line
+    // weights from the cell fractions (pnx_problem_set_cell_fractions and
+    // pnx_problem_set_absorption on the request's own buffers)
+    r.cell_w = base + o.cell;
+    r.abs = base + o.abs;
+    r.mu = base + o.mu;
+    r.ac = base + o.ac;
+    r.grad = base + o.grad;
+    r.extra = base + o.extra;
+    AbsorptionGeometry& ag = r.ag;
+    for (int a = 0; a < 3; ++a) {
+      ag.points[a] = r.abs + r.abs_pts[a];
+      ag.edges[a] = r.abs + r.abs_edges[a];
+      ag.n_boxes[a] = r.d.M[a];
+      ag.inv_span[a] = r.inv_span[a];
+      ag.detector[a] = r.detector[a];
+    }
+    ag.beam_x0 = r.beam_x0;
+    const Geometry& g = r.g;
+    if (launch_material_densities(g.L, g.class_n, g.rho_off, g.T.n, r.cell_w, r.abs + r.abs_rp, r.rho, st) !=
+            hipSuccess ||
+        launch_absorption(ag, g.L, r.cell_w, r.abs + r.abs_emb, r.abs + r.abs_rp, r.abs + r.abs_mac,
+                          r.rho + g.rho_off[0], r.abs + r.abs_am, reinterpret_cast<const int*>(r.abs + r.abs_el),
+                          r.mu, r.ac, r.W, st) != hipSuccess)
+      return -1;
+  }
   r.pos = march_start();
   r.ops_left = march_ops(r.d.n_steps, r.g.nck);
   return 0;
@@ -742,6 +803,22 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
         hipMemcpyAsync(r->pinned_out + nI + nrb, r->gen, ngen * sizeof(double), hipMemcpyDeviceToHost, st) !=
             hipSuccess)
       return -1;
+    if (r->spec && r->grad_out) {
+      // the material pullbacks of the reference adjoint (pn_intensities.jl:406-410)
+      // as pnx_problem_material_gradient: numbers of atoms, densities, absorption
+      const Geometry& g = r->g;
+      const long gs = (long)r->d.n_lines * g.L.V;
+      const double* am = r->abs + r->abs_am;
+      const int* el = reinterpret_cast<const int*>(r->abs + r->abs_el);
+      if (launch_atoms_bar(r->ag, g.L, r->ac, r->gen, gs, 1, am, el, r->extra, st) != hipSuccess ||
+          launch_material_pullback(g.L, g.class_n, g.rho_off, g.T.n, r->cell_w, r->abs + r->abs_rp, r->rb,
+                                   g.rho_total, 1, r->extra, r->grad, st) != hipSuccess ||
+          launch_absorption_pullback(r->ag, g.L, r->cell_w, r->abs + r->abs_rp, r->abs + r->abs_mac, r->ac,
+                                     r->rho + g.rho_off[0], am, el, r->gen, gs, 1, r->grad, st) != hipSuccess ||
+          hipMemcpyAsync(r->pinned_out + nI + nrb + ngen, r->grad, r->sz.grad * sizeof(double),
+                         hipMemcpyDeviceToHost, st) != hipSuccess)
+        return -1;
+    }
     r->done_tick = ln.tick;
   }
   return 0;
@@ -779,6 +856,7 @@ void finish_request(pnx_server* s, Request& r) {
     }
   }
   if (r.gen_out) unpack_lines(r.g, out + nI + nrb, r.gen_out);
+  if (r.spec && r.grad_out) std::memcpy(r.grad_out, out + nI + nrb + r.sz.gen, r.sz.grad * sizeof(double));
   s->pinned_pool.release(r.pinned, r.pinned_bytes);
   r.pinned = nullptr;
   r.pinned_out = nullptr;
@@ -1050,6 +1128,41 @@ int pnx_march_sequence(int32_t n, int32_t K, int32_t* out, int32_t capacity) {
   return ops;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Lane, memory layouts (checkpointed and taped) and sizes of a request whose
+// descriptor, geometry and buffer sizes are set.
+int prepare_request(pnx_server* s, Request& r) {
+  r.lane = lane_of(r.d.N, r.g.L.dims, s->n_lanes - 1);
+  r.off_ = request_offsets(r.sz);
+  r.off_tape = request_offsets(taped_sizes(r.sz, r.d.n_steps));
+  r.bytes = (size_t)r.off_.total * sizeof(double);
+  r.bytes_tape = (size_t)r.off_tape.total * sizeof(double);
+  if ((int64_t)r.bytes > s->budget) return set_error(PNX_ERR_MEMORY, "request exceeds the server's HBM budget");
+  return PNX_OK;
+}
+
+// Queues a prepared request in arrival order and returns its ticket.
+int enqueue(pnx_server* s, const std::shared_ptr<Request>& r, int64_t* ticket) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next_ticket++;
+    auto it = s->pending.end();
+    while (it != s->pending.begin() && (*std::prev(it))->timing.arrival_s > r->timing.arrival_s) --it;
+    s->pending.insert(it, r);
+    s->all[r->ticket] = r;
+    *ticket = r->ticket;
+  }
+  s->cv.notify_all();
+  return PNX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   if (!s || !q || !ticket) return set_error(PNX_ERR_ARG, "null argument");
   if (q->desc.n_beams != 1) return set_error(PNX_ERR_ARG, "served requests carry exactly one beam");
@@ -1062,13 +1175,8 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   int rc = make_geometry(r->d, r->g, err);
   if (rc) return set_error(rc, err);
   r->sz = request_sizes(r->g, r->d);
-  r->lane = lane_of(r->d.N, r->g.L.dims, s->n_lanes - 1);
-  r->off_ = request_offsets(r->sz);
-  r->off_tape = request_offsets(taped_sizes(r->sz, r->d.n_steps));
+  if ((rc = prepare_request(s, *r))) return rc;
   const Offsets& o = r->off_;
-  r->bytes = (size_t)o.total * sizeof(double);
-  r->bytes_tape = (size_t)r->off_tape.total * sizeof(double);
-  if ((int64_t)r->bytes > s->budget) return set_error(PNX_ERR_MEMORY, "request exceeds the server's HBM budget");
   const Geometry& g = r->g;
   for (int c = 0; c < 8; ++c) {
     if (!g.active(c)) continue;
@@ -1100,18 +1208,103 @@ int pnx_server_submit(pnx_server* s, const pnx_request* q, int64_t* ticket) {
   for (int c = 0; c < 8; ++c) r->rho_bar[c] = q->rho_bar[c];
   r->gen_out = q->gen_out;
   r->timing.arrival_s = q->arrival_s;
+  return enqueue(s, r, ticket);
+}
+
+int pnx_server_submit_spec(pnx_server* s, const pnx_request_spec* q, int64_t* ticket) {
+  if (!s || !q || !ticket) return set_error(PNX_ERR_ARG, "null argument");
+  if (q->desc.n_beams != 1) return set_error(PNX_ERR_ARG, "served requests carry exactly one beam");
+  if (!q->physics || !q->cell_w || !q->rho_pure || !q->embedding || !q->mac || !q->atom_mass || !q->line_element ||
+      !q->I_bar || !q->I_out)
+    return set_error(PNX_ERR_ARG, "tables, fractions, absorption inputs, seed and output are required");
+  auto r = std::make_shared<Request>();
+  r->d = q->desc;
+  if (r->d.ckpt_every <= 0) r->d.ckpt_every = default_ckpt(r->d.n_steps);
+  std::string err;
+  int rc = make_geometry(r->d, r->g, err);
+  if (rc) return set_error(rc, err);
+  const pnx_desc& d = r->d;
+  const int ne = d.n_elements, nl = d.n_lines;
+  for (int e = 0; e < ne; ++e)
+    if (!(q->rho_pure[e] > 0.0)) return set_error(PNX_ERR_ARG, "pure-element densities must be positive");
+  for (int l = 0; l < nl; ++l)
+    if (q->line_element[l] < 0 || q->line_element[l] >= ne)
+      return set_error(PNX_ERR_ARG, "line element index out of range");
+  r->spec = true;
+  r->sz = request_sizes(r->g, d, true);
+  if ((rc = prepare_request(s, *r))) return rc;
+  const Geometry& g = r->g;
+  const Offsets& o = r->off_;
+  const size_t nout = (size_t)(nl + r->sz.rb + r->sz.gen + (q->grad_out ? r->sz.grad : 0));
+  const size_t nh = (size_t)o.in_end + nout;
+  r->pinned_bytes = nh * sizeof(double);
+  r->pinned = s->pinned_pool.alloc(r->pinned_bytes);
+  if (!r->pinned) return set_error(PNX_ERR_MEMORY, "pinned staging allocation failed");
+  double* h = r->pinned;
+  std::memset(h, 0, (size_t)o.in_end * sizeof(double));  // densities and line weights are set on the device
+  auto give_back = [&](int code) {
+    s->pinned_pool.release(r->pinned, r->pinned_bytes);
+    return code;
+  };
+  // records, beam moments and profiles: the library's C++ set-up
+  if ((rc = pnx_problem_records(&d, q->physics, h + o.rec_f, h + o.rec_a, h + o.rec_d))) return give_back(rc);
   {
-    std::lock_guard<std::mutex> lk(s->mu);
-    r->ticket = s->next_ticket++;
-    // keep the pending queue ordered by arrival time
-    auto it = s->pending.end();
-    while (it != s->pending.begin() && (*std::prev(it))->timing.arrival_s > q->arrival_s) --it;
-    s->pending.insert(it, r);
-    s->all[r->ticket] = r;
-    *ticket = r->ticket;
+    double mm[8][16] = {};
+    double* m8[8];
+    std::vector<double> pp[8];
+    double* p8[8];
+    for (int c = 0; c < 8; ++c) {
+      m8[c] = odd_in(c, 0) ? mm[c] : nullptr;
+      pp[c].assign(odd_in(c, 0) ? (size_t)g.class_n[c][1] * g.class_n[c][2] : 0, 0.0);
+      p8[c] = odd_in(c, 0) ? pp[c].data() : nullptr;
+    }
+    if ((rc = pnx_beam_moments(d.N, q->beam_direction, q->beam_kappa, m8)) ||
+        (rc = pnx_beam_profiles(&d, q->beam_position, q->beam_sigma, q->literal_source, p8)))
+      return give_back(rc);
+    pack_profiles(g, p8, h + o.prof);
+    for (int c = 0; c < 8; ++c)
+      if (odd_in(c, 0) && g.active(c))
+        for (int m = 0; m < g.T.n[c]; ++m) h[o.mms + c * 16 + m] = mm[c][m];
   }
-  s->cv.notify_all();
-  return PNX_OK;
+  for (int l = 0; l < nl; ++l) h[o.ibar + l] = q->I_bar[l];
+  std::memcpy(h + o.cell, q->cell_w, (size_t)r->sz.cell * sizeof(double));
+  // absorption inputs: even points and box edges per axis, then the tables
+  {
+    double* a0 = h + o.abs;
+    long k = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int M = d.M[a];
+      r->abs_pts[a] = k;
+      for (int i = 0; i < M; ++i) a0[k++] = axis_point(d, a, 0, i);
+      r->abs_edges[a] = k;
+      if (M == 1) {
+        a0[k++] = d.lo[a] - 0.5;
+        a0[k++] = d.lo[a] + 0.5;
+      } else {
+        for (int i = 0; i <= M; ++i) a0[k++] = axis_point(d, a, 1, i);
+      }
+      r->inv_span[a] = (double)M / (a0[r->abs_edges[a] + M] - a0[r->abs_edges[a]]);
+    }
+    r->abs_mac = k;
+    for (int i = 0; i < nl * ne; ++i) a0[k++] = q->mac[i];
+    r->abs_am = k;
+    for (int e = 0; e < ne; ++e) a0[k++] = q->atom_mass[e];
+    r->abs_emb = k;
+    for (int e = 0; e < ne; ++e) a0[k++] = q->embedding[e];
+    r->abs_rp = k;
+    for (int e = 0; e < ne; ++e) a0[k++] = q->rho_pure[e];
+    r->abs_el = k;
+    std::memcpy(a0 + k, q->line_element, nl * sizeof(int32_t));
+  }
+  for (int a = 0; a < 3; ++a) r->detector[a] = q->detector[a];
+  r->beam_x0 = q->beam_position[0];
+  r->pinned_out = h + o.in_end;
+  r->I_out = q->I_out;
+  for (int c = 0; c < 8; ++c) r->rho_bar[c] = q->rho_bar[c];
+  r->gen_out = q->gen_out;
+  r->grad_out = q->grad_out;
+  r->timing.arrival_s = q->arrival_s;
+  return enqueue(s, r, ticket);
 }
 
 int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t) {

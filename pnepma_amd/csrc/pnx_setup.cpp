@@ -203,17 +203,8 @@ int pnx_beam_profiles(const pnx_desc* d, const double* beam_position, double sig
   Geometry g;
   std::string err;
   if (int rc = make_geometry(*d, g, err)) return set_error(rc, err);
-  // axis points as numpy.linspace computes them (the reference's
-  // points(eo, gspec), starmap/src/staggered_grid.jl:67-78)
-  auto point = [&](int a, int odd, int i) {
-    const int M = d->M[a];
-    if (M == 1) return d->lo[a];
-    const double h = (d->hi[a] - d->lo[a]) / (M - 1);
-    const double lo = odd ? d->lo[a] - h / 2 : d->lo[a], hi = odd ? d->hi[a] + h / 2 : d->hi[a];
-    const int n = odd ? M + 1 : M;
-    if (i == n - 1) return hi;
-    return i * ((hi - lo) / (n - 1)) + lo;
-  };
+  // axis points of the reference's points(eo, gspec) (starmap/src/staggered_grid.jl:67-78)
+  auto point = [&](int a, int odd, int i) { return axis_point(*d, a, odd, i); };
   const double norm = std::pow(2.0 * kPi * sigma * sigma, -1.5);
   const double x0 = point(0, 0, 0), xb = beam_position[0] - x0;
   for (int c = 0; c < 8; ++c) {

@@ -23,7 +23,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import Request, Timing, check, dptr, lib
+from ._lib import Request, RequestSpec, Timing, check, dptr, lib
 from .grid import GridSpec
 from .material import CellFractions, MassFractionMaterial
 from .physics import XRay, element
@@ -158,6 +158,65 @@ def prepare(spec: Spec, want_rho_bar: bool = False) -> _Prepared:
     return _Prepared(spec, solve_inputs(spec.problem, spec.material, spec.detector), want_rho_bar)
 
 
+class SpecRequest:
+    """A request as the library takes it with pnx_server_submit_spec: geometry,
+    element tables, cell fractions and absorption inputs (no host-side
+    densities, absorption or records: the library builds those), plus the
+    output arrays.  Only array views and small tables are made here."""
+
+    def __init__(self, spec: Spec, want_gradients: bool = False, want_material_gradient: bool = True):
+        from .physics import mass_absorption
+        from .problem import el_index
+        p, mat = spec.problem, spec.material
+        par = mat.param
+        if not hasattr(par, "vals") or par.vals.shape[:3] != tuple(a.M for a in p.gspec.axes):
+            raise ValueError("a spec request needs one box per grid cell")
+        self.spec = spec
+        ne = len(mat.elements)
+        self.ph, self._keep = p.physics_tables()
+        self.cell_w = np.ascontiguousarray(par.vals.transpose(2, 1, 0, 3).reshape(-1, ne))
+        self.rho_pure = np.ascontiguousarray(mat.rho_pure, dtype=np.float64)
+        self.embedding = np.ascontiguousarray(par.embedding, dtype=np.float64)
+        self.mac = np.ascontiguousarray([[mass_absorption(x, e) for e in mat.elements] for x in p.x_rays],
+                                        dtype=np.float64)
+        self.atom_mass = np.ascontiguousarray([e.atom_mass_g for e in mat.elements], dtype=np.float64)
+        self.line_element = np.ascontiguousarray(el_index(mat.elements, p.x_rays), dtype=np.int32)
+        self.I_bar = np.ascontiguousarray(spec.I_bar, dtype=np.float64)
+        self.I_out = np.zeros(len(p.x_rays))
+        self.rho_bar, self.gen = [None] * 8, None
+        if want_gradients:
+            self.rho_bar = [np.zeros((int(np.prod(p.gspec.class_size(c))), ne)) for c in range(8)]
+            self.gen = np.zeros((int(np.prod(p.gspec.class_size(0))), len(p.x_rays)))
+        self.grad = np.zeros(self.cell_w.shape) if want_material_gradient else None
+        r = RequestSpec()
+        r.desc = make_desc(p, 0)
+        r.physics = C.pointer(self.ph)
+        r.cell_w, r.rho_pure, r.embedding = dptr(self.cell_w), dptr(self.rho_pure), dptr(self.embedding)
+        r.mac, r.atom_mass = dptr(self.mac), dptr(self.atom_mass)
+        r.line_element = self.line_element.ctypes.data_as(C.POINTER(C.c_int32))
+        for a in range(3):
+            r.detector[a] = float(spec.detector[a])
+            r.beam_position[a] = float(p.beam_positions[0][a])
+        direction, kappa = p.beam_direction
+        d = np.asarray(direction, dtype=np.float64)
+        for a in range(3):
+            r.beam_direction[a] = float(d[a])
+        r.beam_kappa = float(kappa)
+        r.beam_sigma = float(p.beam_sigma)
+        r.literal_source = int(p.source_mode == "literal")
+        r.I_bar, r.I_out = dptr(self.I_bar), dptr(self.I_out)
+        for c in range(8):
+            r.rho_bar[c] = dptr(self.rho_bar[c])
+        r.gen_out, r.grad_out = dptr(self.gen), dptr(self.grad)
+        self.req = r
+
+    def material_gradient(self) -> np.ndarray:
+        """The served material gradient, flattened like CellFractions.params."""
+        nx, ny, nz = (a.M for a in self.spec.problem.gspec.axes)
+        ne = self.cell_w.shape[1]
+        return self.grad.reshape(nz, ny, nx, ne).transpose(2, 1, 0, 3).ravel()
+
+
 class Server:
     def __init__(self, hbm_budget_bytes: int, max_active: int = 4096, ctx: Context | None = None):
         self.ctx = ctx or default_context()
@@ -192,6 +251,14 @@ class Server:
         t = C.c_int64()
         check(lib().pnx_server_submit(self.handle, C.byref(r), C.byref(t)))
         self._live[t.value] = prep
+        return t.value
+
+    def submit_spec(self, req: SpecRequest, arrival_s: float = 0.0) -> int:
+        """Submit a request given as cell fractions; the library prepares it in C++."""
+        req.req.arrival_s = float(arrival_s)
+        t = C.c_int64()
+        check(lib().pnx_server_submit_spec(self.handle, C.byref(req.req), C.byref(t)))
+        self._live[t.value] = req
         return t.value
 
     def wait(self, ticket: int) -> Timing:

@@ -162,3 +162,38 @@ def test_cancel_pending_and_admitted():
     for spec, pr in zip(specs[2:], preps[2:]):
         I, rb, gen = _alone(spec)
         assert rel_err(pr.I_out, I) < 1e-12
+
+
+def test_spec_requests_match_prepared_requests_and_the_oracle(oracle_lib):
+    """Requests given as cell fractions (pnx_server_submit_spec: records, beam
+    moments and profiles built by the library's C++ set-up, densities and
+    absorption on the device at admission, the material gradient at
+    completion) against the same requests prepared on the host, and their
+    material gradients against the device problem's and the oracle's."""
+    specs = _specs(n=8, seed=23) + [_spec_3d(3, (10, 8, 6), 26, 4)]
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    reqs = [serving.SpecRequest(s, want_gradients=True) for s in specs]
+    srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=64)
+    tk = [srv.submit(pr, arrival_s=0.001 * n) for n, pr in enumerate(preps)]
+    tk += [srv.submit_spec(rq, arrival_s=0.001 * n) for n, rq in enumerate(reqs)]
+    for t in tk:
+        assert srv.wait(t).status == 0
+    srv.close()
+    for spec, pr, rq in zip(specs, preps, reqs):
+        p = spec.problem
+        e = [rel_err(rq.I_out, pr.I_out)] + [rel_err(rq.rho_bar[c], pr.rho_bar[c]) for c in p.gspec.active_classes()]
+        e.append(rel_err(rq.gen, pr.gen))
+        dp = DeviceProblem(p, None, ckpt_every=0)
+        dp.set_cell_fractions(spec.material, None)
+        dp.set_absorption(spec.material, spec.detector)
+        dp.forward_adjoint(spec.I_bar[None, :], want_rho_bar=False)
+        g_dev = dp.material_gradient()
+        dp.close()
+        e_g = rel_err(rq.material_gradient(), g_dev)
+        mac = np.array([[m for m in row] for row in rq.mac])
+        I_ref, rb, acb, noab = oracle_lib.forward_adjoint(p, pr.si, spec.I_bar)
+        g_ref = oracle_lib.material_gradient(p, spec.material, spec.detector, mac, rb, acb, noab)
+        e_o = rel_err(rq.material_gradient(), g_ref)
+        print(f"N={p.N} M={p.gspec.M}: spec vs prepared {max(e):.2e}, gradient vs device problem {e_g:.2e}, "
+              f"vs oracle {e_o:.2e}, I vs oracle {rel_err(rq.I_out, I_ref):.2e}", flush=True)
+        assert max(e) < 1e-12 and e_g < 1e-11 and e_o < 1e-8 and rel_err(rq.I_out, I_ref) < 1e-10
