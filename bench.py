@@ -52,9 +52,7 @@ METRIC = "served fwd+adj solves/s, p50/p99 latency vs plain GPU loop @ k-ratio e
 
 def _prep_one(spec):
     from pnepma_amd import serving
-    t = time.perf_counter()
-    pr = serving.prepare(spec)
-    return pr, time.perf_counter() - t
+    return serving.prepare(spec, want_rho_bar=True)
 
 
 WORKERS = 16  # the host cores of one GPU's share of the box
@@ -69,19 +67,25 @@ def host_workers():
 
 
 def prepare_stream(n, seed=3):
-    """Draw and prepare the request stream on the host (process pool, before the
-    GPU is touched).  Returns the specs, the prepared requests, the mean
-    preparation time of one request on one core and the pool's wall time."""
-    import multiprocessing as mp
+    """Draw the request stream and build its requests as the library takes them
+    (cell fractions, element tables, absorption inputs: serving.SpecRequest).
+    Returns the specs, the requests and the mean seconds to build one."""
     from pnepma_amd import serving, physics
     for e in physics.ELEMENTS.values():
         physics.transport_coefficients(e, 1000.0, 9)
     specs = serving.serving_requests(n, seed=seed)
     t = time.perf_counter()
+    reqs = [serving.SpecRequest(sp) for sp in specs]
+    return specs, reqs, (time.perf_counter() - t) / n
+
+
+def prepare_host(specs, idx):
+    """Host-side inputs (densities, absorption, records: serving.prepare) of the
+    requests idx, for the oracle (process pool, before the GPU is touched)."""
+    import multiprocessing as mp
     with mp.get_context("fork").Pool(host_workers()) as pool:
-        out = pool.map(_prep_one, specs, chunksize=8)
-    wall = time.perf_counter() - t
-    return specs, [o[0] for o in out], float(sum(o[1] for o in out) / len(out)), wall
+        out = pool.map(_prep_one, [specs[i] for i in idx], chunksize=1)
+    return dict(zip(idx, out))
 
 
 PROBE_BEAMS = 160  # 160 beams x 3.57 MB of state: the working set exceeds the 256 MB Infinity Cache
@@ -129,18 +133,21 @@ def roofline_probe(reps=50):
 
 
 def serve_run(srv, preps, rate):
-    """Submit every prepared request to the running server with Poisson arrivals
-    of the given rate (0: all at once) and wait for all of them; returns (wall
-    seconds from the first arrival to the last completion, latencies in
-    seconds, requests admitted taped)."""
+    """Submit every request (serving.SpecRequest: the library builds its records,
+    beam data, densities and absorption) to the running server with Poisson
+    arrivals of the given rate (0: all at once) and wait for all of them;
+    returns (wall seconds from the first arrival to the last completion,
+    latencies in seconds, requests admitted taped, seconds spent submitting)."""
     import numpy as np
     rng = np.random.default_rng(7)
     gaps = rng.exponential(1.0 / rate, len(preps)) if rate > 0 else np.zeros(len(preps))
     arr = np.cumsum(gaps)
     arr -= arr[0]
     t0 = srv.clock() + 0.05
-    tickets = [srv.submit(pr, arrival_s=t0 + a) for pr, a in zip(preps, arr)]
-    progress(f"submitted {len(tickets)} requests in {srv.clock() - t0 + 0.05:.3f} s")
+    ts = time.perf_counter()
+    tickets = [srv.submit_spec(rq, arrival_s=t0 + a) for rq, a in zip(preps, arr)]
+    submit_s = time.perf_counter() - ts
+    progress(f"submitted {len(tickets)} requests in {submit_s:.3f} s")
     srv.drain()
     tm = [srv.wait(t) for t in tickets]
     lat = np.array([x.done_s - x.arrival_s for x in tm])
@@ -148,7 +155,7 @@ def serve_run(srv, preps, rate):
     bad = sum(1 for x in tm if x.status != 0)
     if bad:
         raise RuntimeError(f"{bad} served requests failed")
-    return wall, lat, sum(x.taped for x in tm)
+    return wall, lat, sum(x.taped for x in tm), submit_s
 
 
 def latency_split(specs, lat, passes):
@@ -163,21 +170,27 @@ def latency_split(specs, lat, passes):
     return out
 
 
-def sequential_run(specs, preps, max_seconds=15.0, max_requests=64):
+def sequential_run(specs, max_seconds=15.0, max_requests=64):
     """The plain loop: one request at a time with the same kernels, no batching.
-    Each request is taped like a served one (its whole forward march kept, no
-    recompute: checkpoint interval = step count), so the ratio to the served
-    rate measures batching alone.  Only forward_adjoint on a prepared device
-    problem is timed."""
+    Each request is set up like a served one (densities and absorption on the
+    device from its cell fractions) and taped like one (its whole forward march
+    kept, no recompute: checkpoint interval = step count), so the ratio to the
+    served rate measures batching alone.  Timed per request: forward_adjoint
+    and the material gradient, on a device problem whose buffers are allocated
+    beforehand."""
     import numpy as np
     from pnepma_amd.solver import DeviceProblem
     times, outs = [], []
     t_all = time.perf_counter()
-    for spec, pr in zip(specs, preps):
-        dp = DeviceProblem(spec.problem, pr.si, ckpt_every=spec.problem.n_steps)
+    for spec in specs:
+        p = spec.problem
+        dp = DeviceProblem(p, None, ckpt_every=p.n_steps)
+        dp.set_cell_fractions(spec.material, None)
+        dp.set_absorption(spec.material, spec.detector)
         dp.reserve_adjoint()  # the server's arena is allocated once too
         t = time.perf_counter()
         I, _, _ = dp.forward_adjoint(spec.I_bar[None, :], want_rho_bar=False)
+        dp.material_gradient()
         times.append(time.perf_counter() - t)
         outs.append(I[0])
         dp.close()
@@ -223,12 +236,13 @@ _CPU = None
 
 
 def _oracle_one(i):
-    """Oracle forward+adjoint of request i (in a pool worker): (i, I, rho_bar, ac_bar, seconds)."""
+    """Oracle forward+adjoint of request i (in a pool worker):
+    (i, I, rho_bar, ac_bar, seconds, noa_bar)."""
     specs, preps = _CPU
     from oracle import oracle
     t = time.perf_counter()
-    I, rb, acb, _ = oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
-    return i, I, rb, acb, time.perf_counter() - t
+    I, rb, acb, noab = oracle.forward_adjoint(specs[i].problem, preps[i].si, specs[i].I_bar)
+    return i, I, rb, acb, time.perf_counter() - t, noab
 
 
 def _median_time(fn, reps=5):
@@ -274,7 +288,7 @@ def cpu_phase(specs, preps, check_idx, tmpdir):
     p, mat, stds, det = workloads.slab1d(n_steps=300)
     si = solve_inputs(p, mat, det)
     sis = [solve_inputs(p, m, det) for m in stds]
-    p.records()
+
     out["slab1d_forward_ms"] = 1e3 * _median_time(lambda: oracle.forward(p, si))
     out["slab1d_kratios_ms"] = 1e3 * _median_time(
         lambda: [oracle.forward(p, si)] + [oracle.forward(p, x) for x in sis])
@@ -303,7 +317,7 @@ def cpu_phase(specs, preps, check_idx, tmpdir):
     wall_all = time.perf_counter() - t
     _CPU = None
     busy = {False: [0.0, 0.0], True: [0.0, 0.0]}
-    for i, _, _, _, el in res:
+    for i, _, _, _, el, _ in res:
         busy[bool(is2d[i])][0] += el
         busy[bool(is2d[i])][1] += cost[i]
     spcn = {k: v[0] / v[1] for k, v in busy.items() if v[1] > 0}
@@ -318,26 +332,33 @@ def cpu_phase(specs, preps, check_idx, tmpdir):
     return out, {r[0]: r for r in res}
 
 
-def served_vs_oracle(specs, preps, ref):
+def served_vs_oracle(specs, reqs, preps, ref):
     """Largest relative differences of the served results from the oracle's:
     intensities, density gradients of every updated class, absorption-factor
-    gradients (from the generation sums)."""
+    gradients (from the generation sums) and the whole material gradient."""
     import numpy as np
+    from oracle import oracle
+    from pnepma_amd import physics
 
     def rel(a, b):
         sc = float(np.max(np.abs(b)))
         return float(np.max(np.abs(a - b))) / sc if sc > 0 else float(np.max(np.abs(a)))
-    eI = erb = eac = 0.0
-    for i, (_, I, rb, acb, _) in ref.items():
-        pr, p = preps[i], specs[i].problem
-        eI = max(eI, rel(pr.I_out, I))
+    eI = erb = eac = eg = 0.0
+    for i, (_, I, rb, acb, _, noab) in ref.items():
+        rq, pr, sp = reqs[i], preps[i], specs[i]
+        p = sp.problem
+        eI = max(eI, rel(rq.I_out, I))
         for c in p.gspec.active_classes():
             if rb[c].size:
-                erb = max(erb, rel(pr.rho_bar[c], rb[c]))
-        eac = max(eac, rel(pr.si.noa[:, pr.si.el_idx] * pr.gen, acb))
+                erb = max(erb, rel(rq.rho_bar[c], rb[c]))
+        eac = max(eac, rel(pr.si.noa[:, pr.si.el_idx] * rq.gen, acb))
+        mac = np.array([[physics.mass_absorption(x, e) for e in sp.material.elements] for x in p.x_rays])
+        g_ref = oracle.material_gradient(p, sp.material, sp.detector, mac, rb, acb, noab)
+        eg = max(eg, rel(rq.material_gradient(), g_ref))
     return {"requests": len(ref), "max_rel_err_intensity": eI, "max_rel_err_rho_bar": erb,
-            "max_rel_err_ac_bar": eac, "tolerance": {"intensity": 1e-10, "gradients": 1e-8},
-            "pass": bool(eI <= 1e-10 and erb <= 1e-8 and eac <= 1e-8),
+            "max_rel_err_ac_bar": eac, "max_rel_err_material_gradient": eg,
+            "tolerance": {"intensity": 1e-10, "gradients": 1e-8},
+            "pass": bool(eI <= 1e-10 and erb <= 1e-8 and eac <= 1e-8 and eg <= 1e-8),
             "sample": "the 8 cheapest requests of each (PN order 3/5/7/9, 1D/2D) stratum of the stream"}
 
 
@@ -350,18 +371,25 @@ def main_serve(args):
     import numpy as np
     t_start = time.perf_counter()
     R, K, W = args.requests, max(1, args.steps), max(0, args.warmup)
-    specs, preps, prep_s, prep_wall = prepare_stream(R)
-    progress(f"prepared {len(preps)} requests: {prep_s * 1e3:.1f} ms each on one core, {prep_wall:.1f} s wall")
+    specs, reqs, build_s = prepare_stream(R)
+    progress(f"drew {R} requests ({build_s * 1e3:.2f} ms each to build as the library takes them)")
     check_idx = check_sample(specs) if not args.no_cpu else []
     for i in check_idx:
-        preps[i].want_gradients()
-    cpu, ref = None, {}
+        reqs[i] = type(reqs[i])(specs[i], want_gradients=True)
+    cpu, ref, preps = None, {}, {}
     if not args.no_cpu:
-        # the C oracle runs before any process touches the GPU
+        # host inputs of the check sample for the oracle, and the C oracle
+        # itself, before any process touches the GPU
+        preps = prepare_host(specs, check_idx)
         with tempfile.TemporaryDirectory() as tmp:
             cpu, ref = cpu_phase(specs, preps, check_idx, tmp)
         progress(f"cpu phase: one core {cpu['one_core_rate']:.4f} solves/s, {cpu['workers']} cores "
                  f"{cpu['all_core_rate']:.3f} solves/s; check sample {cpu['all_core_sample']['wall_s']:.1f} s wall")
+    # torch starts the GPU runtime before this process loads libpnx.so: torch
+    # ships its own HIP runtime, and a process that loaded the system one first
+    # saw no device after torch had started
+    from pnepma_amd import _lib
+    assert _lib._lib is None, "libpnx.so was loaded before the GPU phase"
     import torch
     assert torch.cuda.is_available()
     torch.cuda.init()
@@ -376,15 +404,16 @@ def main_serve(args):
     # warm-up: the head of the stream, W windows of R/K requests (at least 512)
     # all at once; its rate also sets the arrival rate of the timed pass
     n_head = min(R, max(512, W * R // K))
-    w0, _, _ = serve_run(srv, preps[:n_head], 0.0)
+    w0, _, _, _ = serve_run(srv, reqs[:n_head], 0.0)
     capacity = n_head / w0
     rate = 1.5 * capacity
     progress(f"warm-up / capacity probe: {n_head} requests in {w0:.2f} s")
     st0 = srv.stats()
-    # timed: one continuous pass of the whole stream (K windows of R/K requests)
+    # timed: one continuous pass of the whole stream (K windows of R/K
+    # requests); submitting a request runs its C++ set-up in this thread
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    w, lat_sat, taped = serve_run(srv, preps, rate)
+    w, lat_sat, taped, submit_s = serve_run(srv, reqs, rate)
     torch.cuda.synchronize()
     total_wall = time.perf_counter() - t0
     st1 = srv.stats()
@@ -392,22 +421,23 @@ def main_serve(args):
              "peak_active": st1["peak_active"], "peak_bytes": st1["peak_bytes"], "taped": taped}
     served = R / total_wall
     progress(f"timed pass: {R} requests in {total_wall:.2f} s ({served:.1f} solves/s; server {w:.2f} s)")
-    check = served_vs_oracle(specs, preps, ref) if ref else None
+    check = served_vs_oracle(specs, reqs, preps, ref) if ref else None
     if check:
         progress(f"served vs oracle ({check['requests']} requests): I {check['max_rel_err_intensity']:.2e} "
-                 f"rho_bar {check['max_rel_err_rho_bar']:.2e} ac_bar {check['max_rel_err_ac_bar']:.2e}")
+                 f"rho_bar {check['max_rel_err_rho_bar']:.2e} ac_bar {check['max_rel_err_ac_bar']:.2e} "
+                 f"material gradient {check['max_rel_err_material_gradient']:.2e}")
     # latency at a load the server sustains: the first half of the stream at
     # 80 % of the measured throughput (untimed)
     n_lat = R // 2
     rate_lat = 0.8 * served
-    w_lat, lat, taped_lat = serve_run(srv, preps[:n_lat], rate_lat)
+    w_lat, lat, taped_lat, _ = serve_run(srv, reqs[:n_lat], rate_lat)
     srv.close()
     progress(f"latency pass at {rate_lat:.1f}/s: {n_lat} requests in {w_lat:.2f} s, "
              f"p50 {np.percentile(lat, 50) * 1e3:.0f} ms p99 {np.percentile(lat, 99) * 1e3:.0f} ms")
-    seq_t, seq_I = sequential_run(specs, preps)
+    seq_t, seq_I = sequential_run(specs)
     seq_rate = len(seq_t) / seq_t.sum()
     progress(f"plain loop: {len(seq_t)} requests, {seq_rate:.2f} solves/s")
-    alone_err = max(float(np.max(np.abs(a - pr.I_out)) / np.max(np.abs(a))) for a, pr in zip(seq_I, preps))
+    alone_err = max(float(np.max(np.abs(a - rq.I_out)) / np.max(np.abs(a))) for a, rq in zip(seq_I, reqs))
     roof = roofline_probe()
     progress(f"roofline probes done; {time.perf_counter() - t_start:.0f} s since start")
     out = {
@@ -417,15 +447,19 @@ def main_serve(args):
         "config": {"workload": f"serving stream of {R} forward+gradient requests: PN 3/5/7/9, "
                                "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, stable (CFL 0.5) step "
                                "counts, Poisson arrivals at 1.5x the all-at-once capacity of the stream's head; "
-                               f"one timed pass, a step = one window of {R // K} requests",
+                               "each request is submitted as cell fractions and returns its intensities and its "
+                               "material gradient; one timed pass, a step = one window of "
+                               f"{R // K} requests",
                    "requests": R, "arrival_rate_per_s": rate, "hbm_budget_bytes": budget,
                    "tape_limit": "0.95 of the budget (server default)" if args.tape_fraction is None
                    else f"{args.tape_fraction} of the budget",
                    "mean_energy_steps": float(np.mean([s.problem.n_steps for s in specs]))},
         "server_wall_s": w,
-        "host_prep": {"ms_per_request_one_core": prep_s * 1e3, "pool_wall_s": prep_wall, "workers": host_workers(),
-                      "note": "per-request host preparation (densities, absorption ray-march, records, beam "
-                              "moments), outside the timed region"},
+        "host_prep": {"submit_ms_per_request": 1e3 * submit_s / R, "build_ms_per_request": 1e3 * build_s,
+                      "note": "submit = the library's C++ set-up (records, beam moments and profiles) and "
+                              "staging, inside the timed region; build = the Python arrays of a request "
+                              "(cell fractions, element tables), before it; densities and absorption run on "
+                              "the device at admission"},
         "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "latency_load": {"arrival_rate_per_s": rate_lat, "requests": n_lat, "taped": taped_lat,
                          "note": "p50/p99 from an untimed pass of the first half of the stream at 0.8x the "
@@ -435,7 +469,7 @@ def main_serve(args):
                                       "p99": float(np.percentile(lat_sat, 99) * 1e3)},
         "plain_gpu_loop": {"value": seq_rate, "unit": "solves/s", "taped": True,
                            "sample": f"first {len(seq_t)} requests of the stream, one at a time, same kernels, "
-                                     "taped like the server, no batching",
+                                     "taped like the server, material on the device, no batching",
                            "mean_ms": float(seq_t.mean() * 1e3)},
         "speedup_vs_plain_gpu_loop": served / seq_rate,
         "served_vs_alone_max_rel_err": alone_err,

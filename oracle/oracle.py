@@ -107,8 +107,8 @@ def inputs_of(problem, si, beam: int):
     keep.ac = np.ascontiguousarray(si.ac, dtype=np.float64)
     keep.noa = np.ascontiguousarray(si.noa, dtype=np.float64)
     keep.el = np.ascontiguousarray(si.el_idx, dtype=np.int32)
-    keep.mms = problem.beam_moments()
-    keep.ppos = problem.beam_profiles(beam)
+    keep.mms = problem.beam_moments_host()
+    keep.ppos = problem.beam_profiles_host(beam)
     inp = Inputs()
     for c in range(8):
         inp.rho[c] = _ptr(keep.rho[c])
@@ -120,10 +120,20 @@ def inputs_of(problem, si, beam: int):
     return inp, keep
 
 
+def _records(problem):
+    """The oracle's own records of the problem (cached on it).  Together with the
+    numpy beam moments and profiles this keeps the oracle independent of
+    libpnx.so, so it runs in a process that has not loaded the library."""
+    key = "oracle_records"
+    if key not in problem._cache:
+        problem._cache[key] = records(problem)
+    return problem._cache[key]
+
+
 def forward(problem, si, beam: int = 0) -> np.ndarray:
     d = desc_of(problem)
     inp, keep = inputs_of(problem, si, beam)
-    fwd, _, _ = problem.records()
+    fwd, _, _ = _records(problem)
     out = np.zeros(len(problem.x_rays))
     rc = lib().ref_forward(C.byref(d), C.byref(inp), _ptr(fwd), _ptr(out))
     if rc:
@@ -135,7 +145,7 @@ def forward_adjoint(problem, si, I_bar, beam: int = 0):
     """Returns (I, rho_bar[8] as (P_c, n_e), ac_bar (P, r), noa_bar (P, e))."""
     d = desc_of(problem)
     inp, keep = inputs_of(problem, si, beam)
-    fwd, adj, dens = problem.records()
+    fwd, adj, dens = _records(problem)
     ne = len(problem.elements)
     I = np.zeros(len(problem.x_rays))
     rb = [np.zeros((lib().ref_class_points(C.byref(d), c), ne)) for c in range(8)]
@@ -154,8 +164,8 @@ def step(problem, si, u, rec, adjoint=False, q_eee=None, beam: int = 0):
     """One step_pde! on the list of 8 per-class arrays u (modified in place)."""
     d = desc_of(problem)
     rho = (_dp * 8)(*[_ptr(np.ascontiguousarray(r)) for r in si.rho])
-    mms = problem.beam_moments()
-    ppos = problem.beam_profiles(beam)
+    mms = problem.beam_moments_host()
+    ppos = problem.beam_profiles_host(beam)
     m8 = (_dp * 8)(*[_ptr(m) for m in mms])
     p8 = (_dp * 8)(*[_ptr(p) for p in ppos])
     u8 = (_dp * 8)(*[_ptr(x) for x in u])

@@ -208,12 +208,12 @@ def test_source_linearity_exact(oracle_lib):
     si = inputs(p, mat, det)
     I1 = oracle_lib.forward(p, si)
     p.beam_sigma_scale = None
-    orig = p.beam_profiles
+    orig = p.beam_profiles_host
 
     def scaled(b):
         return [None if x is None else x * 2.0 ** -20 for x in orig(b)]
 
-    p.beam_profiles = scaled
+    p.beam_profiles_host = scaled
     I2 = oracle_lib.forward(p, si)
     assert np.array_equal(I2, I1 * 2.0 ** -20)
 
