@@ -82,15 +82,15 @@ int ref_forward(const ref_desc* d, const ref_inputs* in, const double* fwd_rec, 
  * where E_i is the reverse-sweep energy of step i (E_n = E_cutoff).
  * Outputs (accumulated from zero): I_out[r], rho_bar[c] ([point][e] per class),
  * ac_bar[p][r], noa_bar[p][e]. */
+int ref_forward_adjoint(const ref_desc* d, const ref_inputs* in, const double* fwd_rec, const double* adj_rec,
+                        const double* dens_rec, const double* I_bar, double* I_out, double* const rho_bar[8],
+                        double* ac_bar, double* noa_bar);
+
 /* Test infrastructure: plants a known mistake in the adjoint so that a test
  * can show that its check detects it.  1: the adjoint boundary blocks with the
  * wrong sign (+L^T A); 2: the density pullback without its C u term; 3: the
  * adjoint source with the wrong sign.  0 restores the faithful restatement. */
 void ref_set_plant(int k);
-
-int ref_forward_adjoint(const ref_desc* d, const ref_inputs* in, const double* fwd_rec, const double* adj_rec,
-                        const double* dens_rec, const double* I_bar, double* I_out, double* const rho_bar[8],
-                        double* ac_bar, double* noa_bar);
 
 /* One energy step (step_pde!, starmap/src/solver.jl:171-196) on a caller-owned
  * state u[c] ([point][moment] per class, ghost cells included) with the

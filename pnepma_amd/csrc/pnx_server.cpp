@@ -38,6 +38,7 @@
 #include <thread>
 #include <vector>
 
+#include "pnx_admission.hpp"
 #include "pnx_host.hpp"
 #include "pnx_march.hpp"
 
@@ -49,7 +50,6 @@ int set_error(int code, const std::string& msg);
 
 namespace {
 
-constexpr size_t kAlign = 256;
 // Launch phases of a sub-tick, in launch order.  Each request's operation
 // puts one pass in each phase it uses, so the phases are its passes in order,
 // and requests in one phase launch are independent:
@@ -96,52 +96,6 @@ int grid_lanes_for_queues() {
   if (const char* e = std::getenv("GPU_MAX_HW_QUEUES")) q = std::atoi(e);
   return std::max(1, std::min(kMaxLanes - 1, q - 2));
 }
-
-size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
-// First-fit allocator of offsets within a range of `size` bytes.
-class FreeList {
- public:
-  explicit FreeList(size_t size = 0) : size_(size) {
-    if (size) free_[0] = size;
-  }
-  // returns the offset, or SIZE_MAX when no free range is large enough
-  size_t alloc(size_t bytes) {
-    bytes = align_up(bytes);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        const size_t off = it->first, len = it->second;
-        free_.erase(it);
-        if (len > bytes) free_[off + bytes] = len - bytes;
-        used_ += bytes;
-        return off;
-      }
-    }
-    return SIZE_MAX;
-  }
-  void release(size_t off, size_t bytes) {
-    bytes = align_up(bytes);
-    used_ -= bytes;
-    auto it = free_.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  size_t used() const { return used_; }
-  size_t size() const { return size_; }
-
- private:
-  size_t size_ = 0, used_ = 0;
-  std::map<size_t, size_t> free_;
-};
 
 // The device arena: one allocation, sub-allocated first-fit.
 class Arena {
@@ -935,7 +889,10 @@ void worker_loop(pnx_server* s) {
         // checkpointed, but only into a lane short of work: a lane that is
         // busy lets the request wait until memory frees for its tape.
         size_t off = SIZE_MAX;
-        if (s->arena.used() + r->bytes_tape <= (size_t)s->tape_limit) off = s->arena.alloc(r->bytes_tape);
+        const Admission how =
+            admission(s->arena.used(), (size_t)s->tape_limit, r->bytes_tape, lane_n[r->lane],
+                      r->lane == 0 ? kBusyMarchLane : kBusyGridLane);
+        if (how == kAdmitTaped) off = s->arena.alloc(r->bytes_tape);
         if (off != SIZE_MAX) {
           r->off_ = r->off_tape;
           r->bytes = r->bytes_tape;
@@ -943,6 +900,9 @@ void worker_loop(pnx_server* s) {
           r->g.nck = 0;
           r->timing.taped = 1;
         } else {
+          // not taped (or its taped layout found no free range): checkpointed
+          // unless its lane is busy
+          // (kAdmitWait implies a busy lane)
           if (s->tape_limit > 0 && lane_n[r->lane] >= (r->lane == 0 ? kBusyMarchLane : kBusyGridLane)) break;
           off = s->arena.alloc(r->bytes);
         }
