@@ -279,7 +279,7 @@ __global__ void atoms_bar_kernel(AbsGeom g, const double* __restrict__ ac, const
 // rho_bar = sum_r (-ac dx ac_bar)[r] mac[r] is added to that box (boxes found
 // by search on the edges; a sample on the last edge belongs to the last box).
 // ac_bar = noa[el(r)] * gen.  Samples from many points land in one box, so
-// the additions are atomic.
+// the additions are atomic (their rounding depends on their order).
 __global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, const double* __restrict__ rho_pure,
                                        const double* __restrict__ mac, const double* __restrict__ ac,
                                        const double* __restrict__ rho0, const double* __restrict__ atom_mass,
@@ -303,6 +303,20 @@ __global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, 
     const double a_ = -ac[r * g.V + p] * dx * ac_bar;
     for (int e = 0; e < g.n_e; ++e) rb[e] += a_ * mac[r * g.n_e + e];
   }
+  // Consecutive samples of a ray often fall in one box and add the same
+  // value there: each run of samples in one box is added with one atomic
+  // (count times the value) instead of one per sample.
+  long run_cell = -1;
+  int run_len = 0;
+  auto flush = [&]() {
+    if (run_cell < 0) return;
+    const double* wc = w + run_cell * g.n_e;
+    double R = 0.0, dot = 0.0;
+    for (int e = 0; e < g.n_e; ++e) R += wc[e] / rho_pure[e];
+    for (int e = 0; e < g.n_e; ++e) dot += wc[e] * rb[e];
+    const double R_bar = -dot / (R * R);
+    for (int e = 0; e < g.n_e; ++e) atomicAdd(&grad[run_cell * g.n_e + e], run_len * (rb[e] / R + R_bar / rho_pure[e]));
+  };
   for (int s = 0; s < 100; ++s) {
     const double tl = sample_t(s) * lam;
     long cell = 0, mult = 1;
@@ -330,14 +344,15 @@ __global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, 
       cell += (long)(ia < 0 ? 0 : ia) * mult;
       mult *= n;
     }
-    if (!inside) continue;
-    const double* wc = w + cell * g.n_e;
-    double R = 0.0, dot = 0.0;
-    for (int e = 0; e < g.n_e; ++e) R += wc[e] / rho_pure[e];
-    for (int e = 0; e < g.n_e; ++e) dot += wc[e] * rb[e];
-    const double R_bar = -dot / (R * R);
-    for (int e = 0; e < g.n_e; ++e) atomicAdd(&grad[cell * g.n_e + e], rb[e] / R + R_bar / rho_pure[e]);
+    if (!inside) cell = -1;
+    if (cell != run_cell) {
+      flush();
+      run_cell = cell;
+      run_len = 0;
+    }
+    run_len += 1;
   }
+  flush();
 }
 
 }  // namespace

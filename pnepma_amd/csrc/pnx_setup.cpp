@@ -156,9 +156,12 @@ int pnx_beam_moments(int N, const double* direction, double kappa, double* const
   const double nrm = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
   if (!(nrm > 0.0)) return arg_error("beam direction must be nonzero");
   for (double& v : dv) v /= nrm;
-  const double theta = std::acos(std::max(-1.0, std::min(1.0, dv[2])));
-  const double phi = std::atan2(dv[1], dv[0]);
-  const double x = std::cos(theta), sx = std::sin(theta);
+  // The harmonics are evaluated from the direction's components rather than
+  // from its angles: with x = d_z, P_l^m(x) = sin^m(theta) Q_l^m(x) and
+  // sin^m(theta) (cos, sin)(m phi) = (Re, Im) (d_x + i d_y)^m, so a component
+  // that is exactly zero makes the moments that vanish by symmetry exact zeros
+  // (angles would leave rounding residues such as cos(pi/2)).
+  const double x = dv[2];
   // i_l(kappa) / i_0(kappa) by backward recurrence (Miller): f_{l-1} = f_{l+1} + (2l+1)/kappa f_l
   const int top = N + 40 + (int)kappa;
   std::vector<double> f(top + 2, 0.0);
@@ -170,14 +173,22 @@ int pnx_beam_moments(int N, const double* direction, double kappa, double* const
       for (int q = l - 1; q <= top + 1; ++q) f[q] *= 1e-250;
     }
   }
-  // associated Legendre functions without the Condon-Shortley phase
-  double P[kMaxOrder + 1][kMaxOrder + 1] = {};
+  // Q_l^m (associated Legendre functions without the Condon-Shortley phase,
+  // divided by sin^m(theta)): Q_m^m = (2m - 1)!!, then the recurrence in l
+  double Q[kMaxOrder + 1][kMaxOrder + 1] = {};
   for (int m = 0; m <= N; ++m) {
-    double pmm = 1.0;
-    for (int q = 1; q <= m; ++q) pmm *= (2.0 * q - 1.0) * sx;
-    P[m][m] = pmm;
-    if (m + 1 <= N) P[m + 1][m] = x * (2.0 * m + 1.0) * pmm;
-    for (int l = m + 2; l <= N; ++l) P[l][m] = ((2.0 * l - 1.0) * x * P[l - 1][m] - (l + m - 1.0) * P[l - 2][m]) / (l - m);
+    double qmm = 1.0;
+    for (int q = 1; q <= m; ++q) qmm *= 2.0 * q - 1.0;
+    Q[m][m] = qmm;
+    if (m + 1 <= N) Q[m + 1][m] = x * (2.0 * m + 1.0) * qmm;
+    for (int l = m + 2; l <= N; ++l) Q[l][m] = ((2.0 * l - 1.0) * x * Q[l - 1][m] - (l + m - 1.0) * Q[l - 2][m]) / (l - m);
+  }
+  double cm[kMaxOrder + 1], sm[kMaxOrder + 1];  // (d_x + i d_y)^m
+  cm[0] = 1.0;
+  sm[0] = 0.0;
+  for (int m = 1; m <= N; ++m) {
+    cm[m] = cm[m - 1] * dv[0] - sm[m - 1] * dv[1];
+    sm[m] = sm[m - 1] * dv[0] + cm[m - 1] * dv[1];
   }
   const Tables t = make_tables(N);
   for (int c = 0; c < 8; ++c) {
@@ -188,9 +199,9 @@ int pnx_beam_moments(int N, const double* direction, double kappa, double* const
       double fac = 1.0;  // (l - |k|)! / (l + |k|)!
       for (int q = l - ak + 1; q <= l + ak; ++q) fac /= q;
       const double Nlk = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * fac);
-      double Y = Nlk * P[l][ak];
-      if (k > 0) Y *= std::sqrt(2.0) * std::cos(ak * phi);
-      if (k < 0) Y *= std::sqrt(2.0) * std::sin(ak * phi);
+      double Y = Nlk * Q[l][ak];
+      if (k > 0) Y *= std::sqrt(2.0) * cm[ak];
+      if (k < 0) Y *= std::sqrt(2.0) * sm[ak];
       mms[c][i] = -Y * (f[l] / f[0]);
     }
   }

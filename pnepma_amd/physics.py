@@ -244,13 +244,37 @@ def vmf_beam_moments(moments, direction, kappa: float) -> np.ndarray:
     with i_l the modified spherical Bessel function of the first kind.  The
     reference evaluates the same integral with adaptive cubature
     (pn_solver/pn_matrices.jl:331-338).
+
+    Y_lk(mu_hat) is real_sh evaluated from the components of mu_hat instead of
+    its angles: with x = mu_z, P_l^m(x) = sin^m(theta) Q_l^m(x) (Q_m^m = (2m-1)!!
+    and the three-term recurrence in l), and sin^m(theta) (cos, sin)(m phi) =
+    (Re, Im)(mu_x + i mu_y)^m.  A component that is exactly zero then makes the
+    moments that vanish by symmetry exact zeros.
     """
     d = np.asarray(direction, dtype=np.float64)
     d = d / np.linalg.norm(d)
-    theta = math.acos(max(-1.0, min(1.0, d[2])))
-    phi = math.atan2(d[1], d[0])
+    x = float(d[2])
+    L = max((l for l, _ in moments), default=0)
+    Q = np.zeros((L + 1, L + 1))
+    for m in range(L + 1):
+        Q[m, m] = math.prod(2.0 * q - 1.0 for q in range(1, m + 1))
+        if m + 1 <= L:
+            Q[m + 1, m] = x * (2.0 * m + 1.0) * Q[m, m]
+        for l in range(m + 2, L + 1):
+            Q[l, m] = ((2.0 * l - 1.0) * x * Q[l - 1, m] - (l + m - 1.0) * Q[l - 2, m]) / (l - m)
+    z = [complex(1.0, 0.0)]
+    w = complex(float(d[0]), float(d[1]))
+    for m in range(1, L + 1):
+        z.append(z[-1] * w)
     i0 = special.spherical_in(0, kappa)
     out = np.empty(len(moments))
     for n, (l, k) in enumerate(moments):
-        out[n] = -float(real_sh(l, k, theta, phi)) * special.spherical_in(l, kappa) / i0
+        ak = abs(k)
+        Nlk = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - ak) / math.factorial(l + ak))
+        Y = Nlk * Q[l, ak]
+        if k > 0:
+            Y *= math.sqrt(2.0) * z[ak].real
+        elif k < 0:
+            Y *= math.sqrt(2.0) * z[ak].imag
+        out[n] = -Y * special.spherical_in(l, kappa) / i0
     return out

@@ -111,3 +111,30 @@ def test_library_beam_moments_and_profiles(kappa):
                 for x, y in zip(p.beam_profiles(b), p.beam_profiles_host(b)):
                     if x is not None:
                         assert rel_err(x, y) < 1e-14
+
+
+def test_beam_moments_vanishing_by_symmetry_are_exact_zeros():
+    """A beam along an axis is symmetric under the reflections of the other two
+    axes, so every moment odd under one of them vanishes.  Both the library and
+    the numpy formulation evaluate the harmonics from the direction's
+    components and give those moments as exact zeros (from the angles they
+    would carry residues such as cos(pi/2) ~ 6e-17, which the boundary source
+    multiplies by the position pdf)."""
+    from pnepma_amd import grid as G
+    for N in (1, 3, 5, 7, 9):
+        p, _, _, _ = workloads.raster2d(N=N, M=(8, 6), n_steps=10, n_beams=1)
+        cm = G.class_moments(N)
+        for d, odd_axes in (((-1.0, 0.0, 0.0), (1, 2)), ((0.0, 0.0, 1.0), (0, 1)), ((0.0, -2.0, 0.0), (0, 2))):
+            p.beam_direction = (d, 40.0)
+            for mm in (p.beam_moments(), p.beam_moments_host()):
+                for c in range(8):
+                    if mm[c] is None:
+                        continue
+                    for n, (l, k) in enumerate(cm[c]):
+                        # parity of Y_lk under y -> -y: odd for k < 0; under
+                        # z -> -z: (-1)^(l + |k|); under x -> -x: odd for
+                        # k > 0 with |k| odd or k < 0 with |k| even
+                        odd = {1: k < 0, 2: (l + abs(k)) % 2 == 1,
+                               0: (k > 0 and abs(k) % 2 == 1) or (k < 0 and abs(k) % 2 == 0)}
+                        if any(odd[a] for a in odd_axes):
+                            assert mm[c][n] == 0.0, (N, d, c, l, k, mm[c][n])
