@@ -47,6 +47,17 @@ using gd = __attribute__((address_space(1))) double*;
 __device__ __forceinline__ gcd glb(const double* p) { return (gcd)p; }
 __device__ __forceinline__ gd glb(double* p) { return (gd)p; }
 
+// Read-only data that all lanes of a wavefront read at the same address: the
+// energy records, the boundary blocks, the beam moments and the adjoint seed.
+// No kernel writes them, so they are read through the constant address space
+// (4), where a load at a wavefront-uniform address becomes a scalar load.
+// Through a global pointer each such read would be a vector load, and after
+// any store of the kernel the compiler must also wait for each one before
+// the arithmetic that follows, which chains one memory latency per element
+// and degree in the mixing loops.
+using gkd = const __attribute__((address_space(4))) double*;
+__device__ __forceinline__ gkd cst(const double* p) { return (gkd)p; }
+
 template <int N, int C>
 constexpr int ncls() {
   return Tab<N>::t.n[C];
@@ -67,9 +78,9 @@ __device__ __forceinline__ double expm1div(double c) {
 // du += sum over non-singleton axes of A_d[C] (u_F[R] - u_F[L]) / h_d, with
 // boundary ghosts of F rebuilt from this point's own values u (length n_C).
 template <int N, int DIM, int C, bool SRC>
-__device__ __forceinline__ void stencil(const Layout& L, gcd __restrict__ from, gcd __restrict__ coef,
+__device__ __forceinline__ void stencil(const Layout& L, gcd __restrict__ from, gkd coef,
                                         const double* u, double* du, int i, int j, int k, long p, double gfac,
-                                        gcd __restrict__ prof_b, gcd __restrict__ mms) {
+                                        gcd __restrict__ prof_b, gkd mms) {
   constexpr int NC = ncls<N, C>();
   const long V = L.V;
   static_for<0, DIM>([&](auto Dc) {
@@ -94,7 +105,7 @@ __device__ __forceinline__ void stencil(const Layout& L, gcd __restrict__ from, 
         const bool left = idx == 0, right = idx == M - 1;
         if (left || right) {
           constexpr int boff = Tab<N>::t.b_off[F][D];
-          gcd __restrict__ Bm = coef + boff;
+          gkd Bm = coef + boff;
           double pr = 0.0;
           if constexpr (SRC && D == 0) pr = prof_b[F * (long)L.Y * L.Z + j + (long)L.Y * k];
 #pragma unroll
@@ -140,7 +151,7 @@ __device__ __forceinline__ int batch_index(int e0, int b, int n) { return e0 + b
 
 // S and the per-degree C of a point (weighted_sum!, pn_problem.jl:213-220).
 template <int N>
-__device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne, gcd __restrict__ rec, double& S,
+__device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne, gkd rec, double& S,
                                     double* Cl) {
   S = 0.0;
 #pragma unroll
@@ -154,7 +165,7 @@ __device__ __forceinline__ void mix(gcd __restrict__ rc, long V, long p, int ne,
       const int e = e0 + b;
       if (e < ne) {
         S += r[b] * rec[e];
-        gcd __restrict__ ce = rec + ne + e * (N + 1);
+        gkd ce = rec + ne + e * (N + 1);
 #pragma unroll
         for (int l = 0; l <= N; ++l) Cl[l] += r[b] * ce[l];
       }
@@ -188,9 +199,9 @@ __device__ __forceinline__ StepView view_of(const HalfStepArgs& a) {
 // a left ghost 2(-B u) - v_R moves v_R - v_L by 2 B (u1 - u), a right ghost
 // 2(B u + g) - v_L by 2 (B (u1 - u) + g1 - g).
 template <int N, int DIM, int C, bool SRC>
-__device__ __forceinline__ void stencil_ghost_delta(const Layout& L, gcd __restrict__ coef, const double* u,
+__device__ __forceinline__ void stencil_ghost_delta(const Layout& L, gkd coef, const double* u,
                                                     const double* u1, double* du, int i, int j, int k, double gfac,
-                                                    double gfac1, gcd __restrict__ prof_b, gcd __restrict__ mms) {
+                                                    double gfac1, gcd __restrict__ prof_b, gkd mms) {
   constexpr int NC = ncls<N, C>();
   static_for<0, DIM>([&](auto Dc) {
     constexpr int D = decltype(Dc)::value;
@@ -202,7 +213,7 @@ __device__ __forceinline__ void stencil_ghost_delta(const Layout& L, gcd __restr
       const bool left = idx == 0, right = idx == M - 1;
       if (left || right) {
         constexpr int boff = Tab<N>::t.b_off[F][D];
-        gcd __restrict__ Bm = coef + boff;
+        gkd Bm = coef + boff;
         double pr = 0.0;
         if constexpr (SRC && D == 0) pr = prof_b[F * (long)L.Y * L.Z + j + (long)L.Y * k];
         const double ih = L.inv_h[D];
@@ -235,13 +246,13 @@ __device__ __forceinline__ void stencil_ghost_delta(const Layout& L, gcd __restr
 // The update of class C at point p from its stencil result du (the rest of
 // right_hand_side! + update!): un = u + dt * phi * (q - du - C u) / S.
 template <int N, int C, int MODE>
-__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gkd rec, double dt,
                                              const double* u, const double* du, double* un, long p, int beam);
 
 // One update of class C at point p (right_hand_side! + update!,
 // solver_core.jl:92-106, solver.jl:122-124): own values u -> un, in registers.
 template <int N, int DIM, int C, int MODE>
-__device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+__device__ __forceinline__ void update_core(const HalfStepArgs& a, gkd rec, double dt,
                                             gcd __restrict__ from, const double* u, double* un, int i, int j, int k,
                                             long p, int beam) {
   constexpr int NC = ncls<N, C>();
@@ -250,13 +261,13 @@ __device__ __forceinline__ void update_core(const HalfStepArgs& a, gcd __restric
 #pragma unroll
   for (int m = 0; m < NC; ++m) du[m] = 0.0;
   const double gfac = rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, MODE == kForward>(a.L, from, glb(a.coef), u, du, i, j, k, p, gfac,
-                                       glb(a.prof) + beam * a.prof_stride, glb(a.mms));
+  stencil<N, DIM, C, MODE == kForward>(a.L, from, cst(a.coef), u, du, i, j, k, p, gfac,
+                                       glb(a.prof) + beam * a.prof_stride, cst(a.mms));
   apply_update<N, C, MODE>(a, rec, dt, u, du, un, p, beam);
 }
 
 template <int N, int C, int MODE>
-__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+__device__ __forceinline__ void apply_update(const HalfStepArgs& a, gkd rec, double dt,
                                              const double* u, const double* du, double* un, long p, int beam) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
@@ -274,8 +285,8 @@ __device__ __forceinline__ void apply_update(const HalfStepArgs& a, gcd __restri
   if constexpr (MODE == kAdjoint && C == 0) {
     // compute_adjoint_source! (x_ray_intensities.jl:155-170), eee m = 0 only
     if (a.ibar != nullptr) {
-      gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
-      gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+      gkd sig = rec + ne + ne * (N + 1) + 1;
+      gkd ib = cst(a.ibar) + beam * L.n_lines;
       gcd __restrict__ Wp = glb(a.W);
       const int nl = L.n_lines;
       for (int r0 = 0; r0 < nl; r0 += kBatch) {
@@ -326,7 +337,7 @@ __device__ __forceinline__ void stream_store(gd q, double v) {
 // u0 receives the updated zeroth moment of eee (forward even half step), from
 // which the block forms the intensity contributions.
 template <int N, int DIM, int C, int MODE, int PAR>
-__device__ __forceinline__ void update_class(const HalfStepArgs& a, gcd __restrict__ rec, double dt,
+__device__ __forceinline__ void update_class(const HalfStepArgs& a, gkd rec, double dt,
                                              gcd __restrict__ from, gcd src, gd dst, int i, int j, int k, long p,
                                              int beam, double* u0) {
   // src and dst are the same buffer for in-place steps: the own values are
@@ -363,14 +374,14 @@ __device__ __forceinline__ void update_class2(const HalfStepArgs& a, const StepV
     u[m] = stream_load<DIM>(uc + m * V + p);
     du[m] = 0.0;
   }
-  gcd __restrict__ rec = glb(v.rec);
-  gcd __restrict__ rec2 = glb(v.rec2);
+  gkd rec = cst(v.rec);
+  gkd rec2 = cst(v.rec2);
   const double gfac = rec[ne + ne * (N + 1)], gfac2 = rec2[ne + ne * (N + 1)];
   gcd __restrict__ prof_b = glb(a.prof) + beam * a.prof_stride;
-  stencil<N, DIM, C, SRC>(L, from, glb(a.coef), u, du, i, j, k, p, gfac, prof_b, glb(a.mms));
+  stencil<N, DIM, C, SRC>(L, from, cst(a.coef), u, du, i, j, k, p, gfac, prof_b, cst(a.mms));
   apply_update<N, C, MODE>(a, rec, v.dt, u, du, u1, p, beam);
   // the second update's stencil differs from the first only in its boundary ghosts
-  stencil_ghost_delta<N, DIM, C, SRC>(L, glb(a.coef), u, u1, du, i, j, k, gfac, gfac2, prof_b, glb(a.mms));
+  stencil_ghost_delta<N, DIM, C, SRC>(L, cst(a.coef), u, u1, du, i, j, k, gfac, gfac2, prof_b, cst(a.mms));
   apply_update<N, C, MODE>(a, rec2, v.dt2, u1, du, u2, p, beam);
   if (dst) {
     gd out = dst + L.cls_off[C];
@@ -477,7 +488,7 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const Step
         const bool interior = !((odd_in(C, 0) && i == 0) || (DIM >= 2 && odd_in(C, 1) && j == 0) ||
                                 (DIM >= 3 && odd_in(C, 2) && k == 0));
         if (interior)
-          update_class<N, DIM, C, MODE, PAR>(a, glb(v.rec), v.dt, glb(v.from) + bo, glb(v.src) + bo,
+          update_class<N, DIM, C, MODE, PAR>(a, cst(v.rec), v.dt, glb(v.from) + bo, glb(v.src) + bo,
                                              glb(v.dst) + bo, i, j, k, p, beam, &u0);
       }
     });
@@ -559,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void half_step2_kernel(HalfStepArgs a) {
 // pullback of integrate_step_and_add_pullback! at eee points.
 // lam holds the adjoint values of class C at this point (registers).
 template <int N, int DIM, int C>
-__device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ rec, double int_fac,
+__device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int_fac,
                                           gcd __restrict__ u_b, const double* lam, int i, int j, int k, long p,
                                           int beam) {
   constexpr int NC = ncls<N, C>();
@@ -574,8 +585,8 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
     du[m] = 0.0;
   }
   const double gfac = rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, true>(L, u_b, glb(a.coef), u, du, i, j, k, p, gfac, glb(a.prof) + beam * a.prof_stride,
-                          glb(a.mms));
+  stencil<N, DIM, C, true>(L, u_b, cst(a.coef), u, du, i, j, k, p, gfac, glb(a.prof) + beam * a.prof_stride,
+                          cst(a.mms));
   double S, Cl[N + 1];
   mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
   double t[NC];
@@ -585,8 +596,8 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
     t[I] = ((-du[I]) - Cl[l] * u[I]) / S;
   });
   gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
-  gcd __restrict__ s = rec;
-  gcd __restrict__ cl = rec + ne;
+  gkd s = rec;
+  gkd cl = rec + ne;
   const double w = a.L.dvol;
   for (int e0 = 0; e0 < ne; e0 += kBatch) {
     double acc[kBatch];
@@ -607,8 +618,8 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
     }
   }
   if constexpr (C == 0) {
-    gcd __restrict__ sig = rec + ne + ne * (N + 1) + 1;
-    gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+    gkd sig = rec + ne + ne * (N + 1) + 1;
+    gkd ib = cst(a.ibar) + beam * L.n_lines;
     gd __restrict__ g = glb(a.gen) + beam * a.gen_stride;
     const int nl = L.n_lines;
     for (int r0 = 0; r0 < nl; r0 += kBatch) {
@@ -628,7 +639,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gcd __restrict__ re
 }
 
 template <int N, int DIM, int C>
-__device__ __forceinline__ void dens_class(const DensArgs& a, gcd __restrict__ rec, double int_fac,
+__device__ __forceinline__ void dens_class(const DensArgs& a, gkd rec, double int_fac,
                                            gcd __restrict__ u_b, gcd __restrict__ lam_b, int i, int j, int k, long p,
                                            int beam) {
   constexpr int NC = ncls<N, C>();
@@ -676,29 +687,29 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
         lin[m] = l0[m];
         du[m] = 0.0;
       }
-      stencil<N, DIM, C, false>(L, lam_b, glb(a.adj.coef), lin, du, i, j, k, p, 0.0, nullptr, nullptr);
-      apply_update<N, C, kAdjoint>(a.adj, glb(a.rec3), a.adj.dt, lin, du, l0, p, beam);
+      stencil<N, DIM, C, false>(L, lam_b, cst(a.adj.coef), lin, du, i, j, k, p, 0.0, nullptr, nullptr);
+      apply_update<N, C, kAdjoint>(a.adj, cst(a.rec3), a.adj.dt, lin, du, l0, p, beam);
       if (a.a1) {
-        stencil_ghost_delta<N, DIM, C, false>(L, glb(a.adj.coef), lin, l0, du, i, j, k, 0.0, 0.0, nullptr, nullptr);
-        apply_update<N, C, kAdjoint>(a.adj, glb(a.adj.rec), a.adj.dt, l0, du, l1, p, beam);
+        stencil_ghost_delta<N, DIM, C, false>(L, cst(a.adj.coef), lin, l0, du, i, j, k, 0.0, 0.0, nullptr, nullptr);
+        apply_update<N, C, kAdjoint>(a.adj, cst(a.adj.rec), a.adj.dt, l0, du, l1, p, beam);
       } else {
 #pragma unroll
         for (int m = 0; m < NC; ++m) l1[m] = l0[m];
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, l1[m]);
-      dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
+      dens_core<N, DIM, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
       return;
     }
   }
   if (a.rec3 != nullptr) {
-    update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.rec3), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+    update_core<N, DIM, C, kAdjoint>(a.adj, cst(a.rec3), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
 #pragma unroll
     for (int m = 0; m < NC; ++m) l0[m] = l1[m];
   }
-  dens_core<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
+  dens_core<N, DIM, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
   if (a.a1) {
-    update_core<N, DIM, C, kAdjoint>(a.adj, glb(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
+    update_core<N, DIM, C, kAdjoint>(a.adj, cst(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
   } else {
 #pragma unroll
     for (int m = 0; m < NC; ++m) l1[m] = l0[m];
@@ -725,7 +736,7 @@ __device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
         if constexpr (parity_of(C) == 1)
           rev_class<N, DIM, C>(a, lam_b, u_b, i, j, k, p, beam);
         else
-          dens_class<N, DIM, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
+          dens_class<N, DIM, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
       }
     }
   });
@@ -771,7 +782,7 @@ __device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int item
     if constexpr (parity_of(C) == 1)
       rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
     else
-      dens_class<N, 2, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam);
+      dens_class<N, 2, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam);
   });
 }
 
@@ -819,7 +830,7 @@ __device__ __forceinline__ void rev_body_cw3(const RevArgs& a, int beam, int ite
           if constexpr (parity_of(C) == 1)
             rev_class<N, 3, C>(a, lam_b, u_b, i, j, k, p, beam);
           else
-            dens_class<N, 3, C>(a.dens, glb(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
+            dens_class<N, 3, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
         }
       }
     }
@@ -915,7 +926,7 @@ __device__ __forceinline__ double dexpm1div(double x) {
 
 // acc[NF] += sign/h * A_D[C]^T w  for one neighbour point of class C
 template <int N, int C, int D>
-__device__ __forceinline__ void at_w(gcd __restrict__ coef, gcd __restrict__ wc, long V, long q,
+__device__ __forceinline__ void at_w(gkd coef, gcd __restrict__ wc, long V, long q,
                                      double scale, double* acc) {
   constexpr int NC = ncls<N, C>();
   constexpr int F = switch_axis(C, D);
@@ -955,12 +966,12 @@ __device__ __forceinline__ void disc_b(const DiscArgs& a, int beam, int i, int j
       const double ih = L.inv_h[D];
       gcd __restrict__ wc = glb(a.w_in) + bo + L.cls_off[C];
       if constexpr (odd_in(C, D)) {  // F even along D: read by C at iq (as R) and iq+1 (as L)
-        if (iq >= 1) at_w<N, C, D>(glb(a.coef), wc, V, p, -ih, acc);
-        if (iq + 1 <= M - 1) at_w<N, C, D>(glb(a.coef), wc, V, p + st, ih, acc);
+        if (iq >= 1) at_w<N, C, D>(cst(a.coef), wc, V, p, -ih, acc);
+        if (iq + 1 <= M - 1) at_w<N, C, D>(cst(a.coef), wc, V, p + st, ih, acc);
       } else {  // F odd along D (iq in 1..M-1): read by C at iq (as L) and iq-1 (as R); ghosts double
         const double cl = iq == M - 1 ? 2.0 : 1.0, cr = iq == 1 ? 2.0 : 1.0;
-        at_w<N, C, D>(glb(a.coef), wc, V, p, cl * ih, acc);
-        at_w<N, C, D>(glb(a.coef), wc, V, p - st, -cr * ih, acc);
+        at_w<N, C, D>(cst(a.coef), wc, V, p, cl * ih, acc);
+        at_w<N, C, D>(cst(a.coef), wc, V, p - st, -cr * ih, acc);
       }
     }
   });
@@ -984,10 +995,10 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
     u[m] = uc[m * V + p];
     d[m] = 0.0;
   }
-  gcd __restrict__ rec = glb(a.rec);
+  gkd rec = cst(a.rec);
   const double gfac = rec[ne + ne * (N + 1)];
-  stencil<N, DIM, C, true>(L, glb(a.s_from) + bo, glb(a.coef), u, d, i, j, k, p, gfac,
-                          glb(a.prof) + beam * a.prof_stride, glb(a.mms));
+  stencil<N, DIM, C, true>(L, glb(a.s_from) + bo, cst(a.coef), u, d, i, j, k, p, gfac,
+                          glb(a.prof) + beam * a.prof_stride, cst(a.mms));
   double S, Cl[N + 1];
   mix<N>(glb(a.rho) + a.rho_off[C], V, p, ne, rec, S, Cl);
   gd lc = glb(a.lam) + bo + L.cls_off[C];
@@ -1030,7 +1041,7 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
         for (int I = 0; I < NC; ++I) {
           double bt = 0.0;
 #pragma unroll
-          for (int J = 0; J < NF; ++J) bt += glb(a.coef)[boff + J * NC + I] * at[J];
+          for (int J = 0; J < NF; ++J) bt += cst(a.coef)[boff + J * NC + I] * at[J];
           lam[I] -= s2 * bt;
         }
       }
@@ -1045,7 +1056,7 @@ __device__ __forceinline__ void disc_a(const DiscArgs& a, int beam, int i, int j
   gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rho_off[C];
   for (int e = 0; e < ne; ++e) {
     const double se = rec[e];
-    gcd __restrict__ ce = rec + ne + e * (N + 1);
+    gkd ce = rec + ne + e * (N + 1);
     double acc = 0.0;
     static_for<0, NC>([&](auto Ic) {
       constexpr int I = decltype(Ic)::value;
@@ -1075,8 +1086,8 @@ __device__ __forceinline__ void disc_body(const DiscArgs& a, int beam, int tile)
     const long V = L.V;
     const long bo = (long)beam * L.beam_stride;
     const int ne = L.n_e;
-    gcd __restrict__ sig = glb(a.rec) + ne + ne * (N + 1) + 1;
-    gcd __restrict__ ib = glb(a.ibar) + beam * L.n_lines;
+    gkd sig = cst(a.rec) + ne + ne * (N + 1) + 1;
+    gkd ib = cst(a.ibar) + beam * L.n_lines;
     const double u0 = glb(a.u_direct)[bo + L.cls_off[0] + p];
     double add = 0.0;
     gd __restrict__ g = glb(a.gen) + beam * a.gen_stride;
