@@ -299,7 +299,10 @@ int pnx_problem_set_material(pnx_problem* p, const double* const rho[8], const d
   HIP_TRY(hipMemcpyAsync(p->rho.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(p->W.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
+  // densities per class again: the kernels read each class's own array
+  if (p->have_cells) destroy_graph(p);
   p->have_mat = true;
+  p->have_cells = false;
   return PNX_OK;
 }
 
@@ -347,7 +350,7 @@ int pnx_problem_cell_fraction_gradient(pnx_problem* p, const double* extra_eee, 
     HIP_TRY(hipMemcpyAsync(p->extra.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
   }
   HIP_TRY(launch_material_pullback(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho_bar_b.p,
-                                   p->rb_stride, p->d.n_beams, extra_eee ? p->extra.p : nullptr, p->grad.p, s));
+                                   p->rb_stride, p->d.n_beams, extra_eee ? p->extra.p : nullptr, false, p->grad.p, s));
   HIP_TRY(hipMemcpyAsync(grad, p->grad.p, n_cells * ne * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;
@@ -438,11 +441,13 @@ int pnx_problem_material_gradient(pnx_problem* p, double* grad) {
   const double* am = p->abs_small.p + p->abs_off_am;
   HIP_TRY(launch_fill(p->extra.p, (long)ne * g.L.V, 0.0, s));
   HIP_TRY(launch_atoms_bar(p->abs, g.L, p->ac.p, p->gen_b.p, gs, p->d.n_beams, am, p->line_el, p->extra.p, s));
+  // the absorption pullback gathers its density gradient per box in grad,
+  // and the material pullback adds the MassFraction pullback of it
+  HIP_TRY(launch_fill(p->grad.p, n_cells * ne, 0.0, s));
+  HIP_TRY(launch_absorption_pullback(p->abs, g.L, p->abs_small.p + p->abs_off_mac, p->ac.p, p->rho.p + g.rho_off[0],
+                                     am, p->line_el, p->gen_b.p, gs, p->d.n_beams, p->grad.p, s));
   HIP_TRY(launch_material_pullback(g.L, g.class_n, g.rho_off, p->T.n, p->cell_w.p, p->rho_pure.p, p->rho_bar_b.p,
-                                   p->rb_stride, p->d.n_beams, p->extra.p, p->grad.p, s));
-  HIP_TRY(launch_absorption_pullback(p->abs, g.L, p->cell_w.p, p->rho_pure.p, p->abs_small.p + p->abs_off_mac, p->ac.p,
-                                     p->rho.p + g.rho_off[0], am, p->line_el, p->gen_b.p, gs, p->d.n_beams, p->grad.p,
-                                     s));
+                                   p->rb_stride, p->d.n_beams, p->extra.p, true, p->grad.p, s));
   HIP_TRY(hipMemcpyAsync(grad, p->grad.p, n_cells * ne * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return PNX_OK;
@@ -481,7 +486,7 @@ HalfStepArgs half_args(pnx_problem* p, int mode, const double* rec, double dt, c
   a.to_src = src;
   a.to_dst = dst;
   a.rho = p->rho.p;
-  for (int c = 0; c < 8; ++c) a.rho_off[c] = p->g.rho_off[c];
+  density_read_offsets(p->g, p->have_cells, a.rho_off);
   a.coef = mode == kForward ? p->coef_f.p : p->coef_a.p;
   a.rec = rec;
   a.prof = p->prof.p;
@@ -619,10 +624,8 @@ DensArgs dens_args(pnx_problem* p, int i, const double* u, double int_fac) {
   a.u = u;
   a.lam = p->lam.p;
   a.rho = p->rho.p;
-  for (int c = 0; c < 8; ++c) {
-    a.rho_off[c] = p->g.rho_off[c];
-    a.rb_off[c] = p->g.rho_off[c];
-  }
+  density_read_offsets(p->g, p->have_cells, a.rho_off);
+  for (int c = 0; c < 8; ++c) a.rb_off[c] = p->g.rho_off[c];
   a.coef = p->coef_f.p;
   a.rec = p->rec_d.p + (long)i * p->g.rec_len;
   a.prof = p->prof.p;

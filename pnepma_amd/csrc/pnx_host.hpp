@@ -38,6 +38,19 @@ struct Geometry {
   bool active(int c) const { return class_active(c, L.dims) && T.n[c] > 0; }
 };
 
+// Offsets at which the kernels read each class's densities.  When the
+// densities come from per-cell mass fractions on boxes that are the grid's
+// cells (pnx_problem_set_cell_fractions, served cell-fraction requests),
+// point (i, j, k) of every class lies in box (i, j, k): odd point i sits on the
+// lower edge of box i, and the kernels update odd points 1..M-1 only.  Every
+// class then has the eee densities at the same index, so all classes read the
+// eee array, and a pass that updates several classes at a point (or the four
+// wavefronts of a reverse block) reads those densities from HBM once.  The
+// values are the same as in each class's own array, so results do not change.
+inline void density_read_offsets(const Geometry& g, bool cell_boxes, long out[8]) {
+  for (int c = 0; c < 8; ++c) out[c] = cell_boxes ? g.rho_off[0] : g.rho_off[c];
+}
+
 // Validates desc and fills g; returns PNX_OK or an error code with message in err.
 int make_geometry(const pnx_desc& d, Geometry& g, std::string& err);
 

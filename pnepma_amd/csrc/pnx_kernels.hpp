@@ -176,14 +176,16 @@ hipError_t launch_rev(int N, int dims, const RevArgs& a, hipStream_t s);
 hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipStream_t s);
 // Device material layer (pnx_material.hip): densities of every active class
 // from per-cell mass fractions w[cell][e], and the pullback of the per-beam
-// density gradient (plus an optional eee term, [e][V]) to grad[cell][e].
+// density gradient (plus an optional eee term, [e][V]) to grad[cell][e];
+// with box_rb_in_grad, grad holds on entry a per-box density gradient
+// (launch_absorption_pullback) whose pullback is added too.
 hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                      const int ncls_of[8], const double* w, const double* rho_pure, double* rho,
                                      hipStream_t s);
 hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                     const int ncls_of[8], const double* w, const double* rho_pure,
                                     const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
-                                    double* grad, hipStream_t s);
+                                    bool box_rb_in_grad, double* grad, hipStream_t s);
 // Absorption factors on the device (pnx_material.hip).  Device pointers:
 // even points and box edges per axis, detector, the plane x = beam_x0 that
 // cuts the rays; inv_span[a] = n_boxes[a] / (last edge - first edge).
@@ -201,10 +203,12 @@ hipError_t launch_absorption(const AbsorptionGeometry& a, const Layout& L, const
 hipError_t launch_atoms_bar(const AbsorptionGeometry& a, const Layout& L, const double* ac, const double* gen_b,
                             long gen_stride, int n_beams, const double* atom_mass, const int* line_el, double* extra,
                             hipStream_t s);
-hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* w,
-                                      const double* rho_pure, const double* mac, const double* ac, const double* rho,
-                                      const double* atom_mass, const int* line_el, const double* gen_b,
-                                      long gen_stride, int n_beams, double* grad, hipStream_t s);
+// Adds the density gradient of the absorption pullback into box_rb[box][e]
+// (zeroed by the caller), for launch_material_pullback with box_rb_in_grad.
+hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* mac,
+                                      const double* ac, const double* rho, const double* atom_mass,
+                                      const int* line_el, const double* gen_b, long gen_stride, int n_beams,
+                                      double* box_rb, hipStream_t s);
 hipError_t launch_fill(double* p, long n, double v, hipStream_t s);
 // dst[i] = src[i], i < n (device to device, as a kernel: no copy-engine round trip)
 hipError_t launch_copy(double* dst, const double* src, long n, hipStream_t s);

@@ -61,9 +61,12 @@ __global__ void densities_kernel(MatGeom g, const double* __restrict__ w, const 
 // One thread per cell: gathers the class points that lie in the cell (at most
 // two per odd axis), sums their density gradient over beams in beam order
 // and adds the MassFraction pullback of each point to the cell's fractions.
+// With box_rb_in_grad, grad holds on entry the density gradient that the
+// absorption pullback gathered per box (absorb_pullback_kernel), whose
+// MassFraction pullback is added last.
 __global__ void pullback_kernel(MatGeom g, const double* __restrict__ w, const double* __restrict__ rho_pure,
                                 const double* __restrict__ rho_bar, long rb_stride, int n_beams,
-                                const double* __restrict__ extra_eee, double* __restrict__ grad) {
+                                const double* __restrict__ extra_eee, int box_rb_in_grad, double* grad) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long n_cells = (long)g.M[0] * g.M[1] * g.M[2];
   if (cell >= n_cells) return;
@@ -103,6 +106,16 @@ __global__ void pullback_kernel(MatGeom g, const double* __restrict__ w, const d
           const double R_bar = -dot / (R * R);
           for (int e = 0; e < ne; ++e) acc[e] += rb[e] / R + R_bar / rho_pure[e];
         }
+  }
+  if (box_rb_in_grad) {
+    double rb[kMaxE];
+    double dot = 0.0;
+    for (int e = 0; e < ne; ++e) {
+      rb[e] = grad[cell * ne + e];
+      dot += wc[e] * rb[e];
+    }
+    const double R_bar = -dot / (R * R);
+    for (int e = 0; e < ne; ++e) acc[e] += rb[e] / R + R_bar / rho_pure[e];
   }
   for (int e = 0; e < ne; ++e) grad[cell * ne + e] = acc[e];
 }
@@ -278,13 +291,17 @@ __global__ void atoms_bar_kernel(AbsGeom g, const double* __restrict__ ac, const
 // at every sample inside the boxes the MassFraction pullback of
 // rho_bar = sum_r (-ac dx ac_bar)[r] mac[r] is added to that box (boxes found
 // by search on the edges; a sample on the last edge belongs to the last box).
-// ac_bar = noa[el(r)] * gen.  Samples from many points land in one box, so
-// the additions are atomic (their rounding depends on their order).
-__global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, const double* __restrict__ rho_pure,
-                                       const double* __restrict__ mac, const double* __restrict__ ac,
+// ac_bar = noa[el(r)] * gen.  That pullback is linear in rho_bar for the
+// box's fixed fractions, so this kernel adds only rho_bar, times the number
+// of samples, into the box's row of box_rb (zeroed; n_e per box), and
+// pullback_kernel applies the MassFraction pullback once per box.  The march
+// then waits for no load of the box's fractions.  Samples from many points
+// land in one box, so the additions are atomic (their rounding depends on
+// their order).
+__global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ mac, const double* __restrict__ ac,
                                        const double* __restrict__ rho0, const double* __restrict__ atom_mass,
                                        const int* __restrict__ line_el, const double* __restrict__ gen_b,
-                                       long gen_stride, int n_beams, double* __restrict__ grad) {
+                                       long gen_stride, int n_beams, double* __restrict__ box_rb) {
 #pragma clang fp contract(off)
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nq = (long)g.M[0] * g.M[1] * g.M[2];
@@ -303,19 +320,13 @@ __global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, 
     const double a_ = -ac[r * g.V + p] * dx * ac_bar;
     for (int e = 0; e < g.n_e; ++e) rb[e] += a_ * mac[r * g.n_e + e];
   }
-  // Consecutive samples of a ray often fall in one box and add the same
-  // value there: each run of samples in one box is added with one atomic
-  // (count times the value) instead of one per sample.
+  // Consecutive samples of a ray often fall in one box: each run of samples
+  // in one box is added with one atomic (count times the value).
   long run_cell = -1;
   int run_len = 0;
   auto flush = [&]() {
     if (run_cell < 0) return;
-    const double* wc = w + run_cell * g.n_e;
-    double R = 0.0, dot = 0.0;
-    for (int e = 0; e < g.n_e; ++e) R += wc[e] / rho_pure[e];
-    for (int e = 0; e < g.n_e; ++e) dot += wc[e] * rb[e];
-    const double R_bar = -dot / (R * R);
-    for (int e = 0; e < g.n_e; ++e) atomicAdd(&grad[run_cell * g.n_e + e], run_len * (rb[e] / R + R_bar / rho_pure[e]));
+    for (int e = 0; e < g.n_e; ++e) atomicAdd(&box_rb[run_cell * g.n_e + e], run_len * rb[e]);
   };
   for (int s = 0; s < 100; ++s) {
     const double tl = sample_t(s) * lam;
@@ -329,16 +340,16 @@ __global__ void absorb_pullback_kernel(AbsGeom g, const double* __restrict__ w, 
       if (x == ed[n]) {
         ia = n - 1;
       } else {
-        // searchsorted(edges, x, side="right") - 1
-        int lo = 0, hi = n + 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (ed[mid] <= x)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        ia = lo - 1;
+        // searchsorted(edges, x, side="right") - 1: the largest ia in [-1, n]
+        // with edges[ia] <= x.  The floor on the uniform spacing starts within
+        // one box of it, and the two corrections move to it on the edges
+        // themselves (the largest such index for any non-decreasing edges,
+        // NaN included: it gives -1), so the result is the search's while
+        // the loads are independent instead of a chain of eight per axis.
+        const double f = floor((x - ed[0]) * g.inv[a]);
+        ia = f >= (double)n ? n : (f >= 0.0 ? (int)f : -1);
+        while (ia < n && ed[ia + 1] <= x) ++ia;
+        while (ia >= 0 && ed[ia] > x) --ia;
       }
       if (ia < 0 || ia > n - 1) inside = false;
       cell += (long)(ia < 0 ? 0 : ia) * mult;
@@ -373,12 +384,12 @@ hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], c
 hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                     const int ncls_of[8], const double* w, const double* rho_pure,
                                     const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
-                                    double* grad, hipStream_t s) {
+                                    bool box_rb_in_grad, double* grad, hipStream_t s) {
   if (L.n_e > kMaxE) return hipErrorInvalidValue;
   MatGeom g = make_mat_geom(L, class_n, rho_off, class_active_dims, ncls_of);
   const long n_cells = (long)L.Mx * L.My * L.Mz;
   hipLaunchKernelGGL(pullback_kernel, dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0, s, g, w, rho_pure,
-                     rho_bar, rb_stride, n_beams, extra_eee, grad);
+                     rho_bar, rb_stride, n_beams, extra_eee, box_rb_in_grad ? 1 : 0, grad);
   return hipGetLastError();
 }
 
@@ -407,14 +418,15 @@ hipError_t launch_atoms_bar(const AbsorptionGeometry& a, const Layout& L, const 
   return hipGetLastError();
 }
 
-hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* w,
-                                      const double* rho_pure, const double* mac, const double* ac, const double* rho,
-                                      const double* atom_mass, const int* line_el, const double* gen_b,
-                                      long gen_stride, int n_beams, double* grad, hipStream_t s) {
+hipError_t launch_absorption_pullback(const AbsorptionGeometry& a, const Layout& L, const double* mac,
+                                      const double* ac, const double* rho, const double* atom_mass,
+                                      const int* line_el, const double* gen_b, long gen_stride, int n_beams,
+                                      double* box_rb, hipStream_t s) {
+  if (L.n_e > kMaxE || L.n_lines > kMaxE) return hipErrorInvalidValue;
   AbsGeom g = abs_geom(a, L);
   const long nq = (long)L.Mx * L.My * L.Mz;
-  hipLaunchKernelGGL(absorb_pullback_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, w, rho_pure,
-                     mac, ac, rho, atom_mass, line_el, gen_b, gen_stride, n_beams, grad);
+  hipLaunchKernelGGL(absorb_pullback_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, g, mac, ac, rho,
+                     atom_mass, line_el, gen_b, gen_stride, n_beams, box_rb);
   return hipGetLastError();
 }
 

@@ -450,7 +450,7 @@ HalfStepArgs make_half(pnx_server* s, const Request& r, int mode, const double* 
   a.to_src = src;
   a.to_dst = dst;
   a.rho = r.rho;
-  for (int c = 0; c < 8; ++c) a.rho_off[c] = r.g.rho_off[c];
+  density_read_offsets(r.g, r.spec, a.rho_off);
   a.coef = coef_for(s, r.d.N, mode == kAdjoint);
   a.rec = rec;
   a.prof = r.prof;
@@ -471,10 +471,8 @@ DensArgs make_dens(pnx_server* s, const Request& r, int i, const double* u, doub
   a.u = u;
   a.lam = r.lam;
   a.rho = r.rho;
-  for (int c = 0; c < 8; ++c) {
-    a.rho_off[c] = r.g.rho_off[c];
-    a.rb_off[c] = r.g.rho_off[c];
-  }
+  density_read_offsets(r.g, r.spec, a.rho_off);
+  for (int c = 0; c < 8; ++c) a.rb_off[c] = r.g.rho_off[c];
   a.coef = coef_for(s, r.d.N, false);
   a.rec = r.rec_d + (long)i * r.g.rec_len;
   a.prof = r.prof;
@@ -764,11 +762,13 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
       const long gs = (long)r->d.n_lines * g.L.V;
       const double* am = r->abs + r->abs_am;
       const int* el = reinterpret_cast<const int*>(r->abs + r->abs_el);
+      const long n_cells = (long)g.L.Mx * g.L.My * g.L.Mz;
       if (launch_atoms_bar(r->ag, g.L, r->ac, r->gen, gs, 1, am, el, r->extra, st) != hipSuccess ||
+          launch_fill(r->grad, n_cells * r->d.n_elements, 0.0, st) != hipSuccess ||
+          launch_absorption_pullback(r->ag, g.L, r->abs + r->abs_mac, r->ac, r->rho + g.rho_off[0], am, el, r->gen,
+                                     gs, 1, r->grad, st) != hipSuccess ||
           launch_material_pullback(g.L, g.class_n, g.rho_off, g.T.n, r->cell_w, r->abs + r->abs_rp, r->rb,
-                                   g.rho_total, 1, r->extra, r->grad, st) != hipSuccess ||
-          launch_absorption_pullback(r->ag, g.L, r->cell_w, r->abs + r->abs_rp, r->abs + r->abs_mac, r->ac,
-                                     r->rho + g.rho_off[0], am, el, r->gen, gs, 1, r->grad, st) != hipSuccess ||
+                                   g.rho_total, 1, r->extra, true, r->grad, st) != hipSuccess ||
           hipMemcpyAsync(r->pinned_out + nI + nrb + ngen, r->grad, r->sz.grad * sizeof(double),
                          hipMemcpyDeviceToHost, st) != hipSuccess)
         return -1;
