@@ -26,7 +26,7 @@ struct Layout {
   int tiles_x, tiles_y;  // tiles per row and per column of one z-plane
   int n_tiles;      // tiles_x * tiles_y * Mz
   int cw_tiles_x;   // reverse passes with one class per wavefront: cw tiles per row
-  int n_cw_items;   // and blocks per beam (4 per cw tile and z-plane; 0 in 1D)
+  int n_cw_items;   // and 64-cell sub-tiles per beam (4 per cw tile and z-plane; 0 in 1D)
   int rec_len;
 };
 

@@ -742,42 +742,55 @@ __device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
   });
 }
 
-// 2D form with one class per wavefront: block `item` takes 64 cells, the
-// rows of sub-tile item & 3 of cw tile item >> 2 (kCwTileX x kCwTileX cells,
-// tiles x-fastest), and wavefront w the class 2w (eee, eoe, oee, ooe) at
-// those cells.  Each wavefront runs one class's code, so the kernel needs
-// the registers of one class instead of all four, which raises the occupancy
-// of these register-heavy passes; a wavefront reads whole row segments of
-// kCwTileX doubles per moment.
+// 2D form with one class per wavefront.  The plane is cut into kCwTileX x
+// kCwTileX cw tiles (x-fastest), each into four 64-cell sub-tiles of
+// 64 / kCwTileX rows; block `pair` takes sub-tiles 2 (pair & 1) and
+// 2 (pair & 1) + 1 of cw tile pair >> 1.  A wavefront runs one class's code at
+// a time, so the kernel needs the registers of one class instead of all four,
+// which raises the occupancy of these register-heavy passes, and it reads
+// whole row segments of kCwTileX doubles per moment.  The odd classes (eoe,
+// oee) take two adjoint half steps and their pullback, the even classes
+// (eee, ooe) the pullback only, so each wavefront takes one job of each kind:
+// wavefronts 0 and 1 the odd class eoe / oee at the first sub-tile and the
+// even class eee / ooe at the second, wavefronts 2 and 3 the same classes at
+// the other sub-tiles.  (With one class per wavefront at one sub-tile the even
+// wavefronts finished early while the block held its place until the odd ones
+// were done.)  The jobs of a block touch disjoint data.
 template <int N, class F>
-__device__ __forceinline__ void class_per_wave_2d(const Layout& L, int item, F&& f) {
+__device__ __forceinline__ void class_pairs_2d(const Layout& L, int pair, F&& f) {
   constexpr int SUB_ROWS = 64 / kCwTileX;
   static_assert(64 % kCwTileX == 0 && kCwTileX == 4 * SUB_ROWS, "a sub-tile is 64 cells, a cw tile four");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tile = item >> 2, sub = item & 3;
+  const int tile = pair >> 1, s0 = 2 * (pair & 1);
   const int tx = tile % L.cw_tiles_x, ty = tile / L.cw_tiles_x;
   const int i = tx * kCwTileX + lane % kCwTileX;
-  const int j = ty * kCwTileX + sub * SUB_ROWS + lane / kCwTileX;
-  if (i >= L.Mx || j >= L.My) return;
-  const long p = i + (long)L.X * j;
-  static_for<0, 4>([&](auto Wc) {
-    constexpr int C = 2 * decltype(Wc)::value;
+  const int j0 = ty * kCwTileX + lane / kCwTileX;
+  const int s_odd = s0 + (wave >> 1), s_even = s0 + 1 - (wave >> 1);
+  auto job = [&](auto Cc, int sub) {
+    constexpr int C = decltype(Cc)::value;
     if constexpr (ncls<N, C>() > 0) {
-      if (wave == decltype(Wc)::value) {
-        const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0));
-        if (interior) f(std::integral_constant<int, C>{}, i, j, p);
-      }
+      const int j = j0 + sub * SUB_ROWS;
+      if (i >= L.Mx || j >= L.My) return;
+      const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0));
+      if (interior) f(Cc, i, j, i + (long)L.X * j);
     }
-  });
+  };
+  if ((wave & 1) == 0) {
+    job(std::integral_constant<int, 2>{}, s_odd);
+    job(std::integral_constant<int, 0>{}, s_even);
+  } else {
+    job(std::integral_constant<int, 4>{}, s_odd);
+    job(std::integral_constant<int, 6>{}, s_even);
+  }
 }
 
 template <int N>
-__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int item) {
+__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int pair) {
   const Layout& L = a.adj.L;
   const long bo = (long)beam * L.beam_stride;
   gcd lam_b = glb(a.adj.from) + bo;
   gcd u_b = glb(a.dens.u) + bo;
-  class_per_wave_2d<N>(L, item, [&](auto Cc, int i, int j, long p) {
+  class_pairs_2d<N>(L, pair, [&](auto Cc, int i, int j, long p) {
     constexpr int C = decltype(Cc)::value;
     if constexpr (parity_of(C) == 1)
       rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
@@ -792,9 +805,9 @@ constexpr int rev_cw_min_blocks(int N) { return N <= 7 ? 3 : 1; }
 
 template <int N>
 __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(RevArgs a) {
-  int beam, item;
-  block_coords(a.adj.L.n_cw_items, a.adj.n_beams, beam, item);
-  rev_body_cw<N>(a, beam, item);
+  int beam, pair;
+  block_coords(a.adj.L.n_cw_items / 2, a.adj.n_beams, beam, pair);
+  rev_body_cw<N>(a, beam, pair);
 }
 
 // 3D form of the fused reverse pass with one class per wavefront: two blocks
@@ -1182,7 +1195,7 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
   const dim3 grid(a.adj.L.n_tiles * a.adj.n_beams), block(kBlock);
   switch (dims) {
     case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(a.adj.L.n_cw_items / 2 * a.adj.n_beams), block, 0, s, a); break;
     case 3:
       hipLaunchKernelGGL((rev_kernel_cw3<N>), dim3(2 * a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a);
       break;
