@@ -66,14 +66,19 @@ def host_workers():
     return max(1, min(WORKERS, n))
 
 
-def prepare_stream(n, seed=3):
+def prepare_stream(n, seed=3, grids="all"):
     """Draw the request stream and build its requests as the library takes them
     (cell fractions, element tables, absorption inputs: serving.SpecRequest).
-    Returns the specs, the requests and the mean seconds to build one."""
+    grids "1d" / "2d" keeps only the stream's requests of that dimension (in
+    stream order), to measure what each kind takes of the GPU.  Returns the
+    specs, the requests and the mean seconds to build one."""
     from pnepma_amd import serving, physics
     for e in physics.ELEMENTS.values():
         physics.transport_coefficients(e, 1000.0, 9)
     specs = serving.serving_requests(n, seed=seed)
+    if grids != "all":
+        specs = [sp for sp in specs if (sp.problem.gspec.axes[1].M > 1) == (grids == "2d")]
+        n = len(specs)
     t = time.perf_counter()
     reqs = [serving.SpecRequest(sp) for sp in specs]
     return specs, reqs, (time.perf_counter() - t) / n
@@ -370,8 +375,9 @@ def main_serve(args):
     import tempfile
     import numpy as np
     t_start = time.perf_counter()
-    R, K, W = args.requests, max(1, args.steps), max(0, args.warmup)
-    specs, reqs, build_s = prepare_stream(R)
+    K, W = max(1, args.steps), max(0, args.warmup)
+    specs, reqs, build_s = prepare_stream(args.requests, grids=args.grids)
+    R = len(specs)
     progress(f"drew {R} requests ({build_s * 1e3:.2f} ms each to build as the library takes them)")
     check_idx = check_sample(specs) if not args.no_cpu else []
     for i in check_idx:
@@ -445,6 +451,8 @@ def main_serve(args):
         "ms_per_step": 1e3 * total_wall / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"serving stream of {R} forward+gradient requests: PN 3/5/7/9, "
+                               + ("" if args.grids == "all" else
+                                  f"only the {args.grids.upper()} requests of a {args.requests}-request stream: ") +
                                "1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements, stable (CFL 0.5) step "
                                "counts, Poisson arrivals at 1.5x the all-at-once capacity of the stream's head; "
                                "each request is submitted as cell fractions and returns its intensities and its "
@@ -621,6 +629,8 @@ def main():
     ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "raster3d_step", "kernel"])
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grids", default="all", choices=["all", "1d", "2d"],
+                    help="serve only the stream's 1D or 2D requests (what each kind takes of the GPU)")
     ap.add_argument("--tape-fraction", type=float, default=None,
                     help="share of the HBM budget the server may hold in taped requests (0: always checkpoint)")
     args = ap.parse_args()

@@ -673,14 +673,15 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
   double l0[NC], l1[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) l0[m] = stream_load<DIM>(lc + m * V + p);
-  if constexpr (N < 9) {
+  if constexpr (N < 9 || DIM == 2) {
     if (a.rec3 != nullptr) {
       // The pullback reads lambda after the last odd half step of step i + 1
       // (l0), and the first odd half step of step i does not read what the
       // pullback writes, so the two odd half steps run back to back as a
       // fused pair (one stencil; the second adds only the change of the
       // boundary ghosts), lambda is stored, and the pullback follows with l0.
-      // (At P9 this order measured 6 % slower than the one below.)
+      // (2D P9: 226 VGPRs against 208 for the order below, both occupancy 2,
+      // and 6 % faster; in 1D and 3D P9 keeps the order below.)
       double lin[NC], du[NC];
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
