@@ -311,7 +311,9 @@ typedef struct {
   const double* I_bar;           /* n_lines pullback seed */
   double arrival_s;              /* admission not before this server-clock time */
   double* I_out;                 /* n_lines, written at completion */
-  double* rho_bar[8];            /* optional [point][element] per class */
+  double* rho_bar[8];            /* optional [point][element] per class; when all are null on a 2D
+                                    request of at most 4 elements, the reverse passes sum a cell's
+                                    classes into one per-cell gradient that only grad_out uses */
   double* gen_out;               /* optional [eee point][line] */
   double* grad_out;              /* optional [cell][element]: d(I_bar . I)/d(cell fractions) */
 } pnx_request_spec;

@@ -97,7 +97,17 @@ struct DensArgs {
   double dE;
   double int_fac;
   int n_beams;
+  // 1: rho_bar is [b][e][V] per cell, at the cell's eee index, and a 2D
+  // reverse block adds the four classes' gradients at a cell, in class order,
+  // with one read and write of it (densities from per-cell fractions on boxes
+  // that are the grid's cells, where the material pullback sums the classes
+  // of a cell anyway; n_e <= kCellE).  0: [b][c][e][V] per class point at
+  // rb_off[c].
+  int rb_cell;
 };
+
+// Elements a per-cell density gradient (DensArgs::rb_cell) can carry.
+constexpr int kCellE = 4;
 
 // Fused reverse pass of one reference-adjoint step (see rev_body).
 // Fused reverse pass of step i: [last odd adjoint half step of step i + 1]
@@ -178,14 +188,16 @@ hipError_t launch_disc(int N, int dims, int pb, int pa, const DiscArgs& a, hipSt
 // from per-cell mass fractions w[cell][e], and the pullback of the per-beam
 // density gradient (plus an optional eee term, [e][V]) to grad[cell][e];
 // with box_rb_in_grad, grad holds on entry a per-box density gradient
-// (launch_absorption_pullback) whose pullback is added too.
+// (launch_absorption_pullback) whose pullback is added too; with rb_cell the
+// density gradient is per cell ([b][e][V] at the cell's eee index,
+// DensArgs::rb_cell) instead of per class point.
 hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                      const int ncls_of[8], const double* w, const double* rho_pure, double* rho,
                                      hipStream_t s);
 hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                     const int ncls_of[8], const double* w, const double* rho_pure,
                                     const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
-                                    bool box_rb_in_grad, double* grad, hipStream_t s);
+                                    bool box_rb_in_grad, double* grad, hipStream_t s, bool rb_cell = false);
 // Absorption factors on the device (pnx_material.hip).  Device pointers:
 // even points and box edges per axis, detector, the plane x = beam_x0 that
 // cuts the rays; inv_span[a] = n_boxes[a] / (last edge - first edge).

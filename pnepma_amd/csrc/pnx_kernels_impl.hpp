@@ -569,10 +569,13 @@ __global__ __launch_bounds__(kBlock) void half_step2_kernel(HalfStepArgs a) {
 // rho_bar += lambda (s du + c u) dvol dE int_fac, plus the x-ray generation
 // pullback of integrate_step_and_add_pullback! at eee points.
 // lam holds the adjoint values of class C at this point (registers).
-template <int N, int DIM, int C>
+// CELL: the density gradient of this class point is returned in cell_acc[e]
+// (e < n_e <= kCellE) for the block to add per cell (DensArgs::rb_cell)
+// instead of being added to the class's own rho_bar.
+template <int N, int DIM, int C, bool CELL = false>
 __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int_fac,
                                           gcd __restrict__ u_b, const double* lam, int i, int j, int k, long p,
-                                          int beam) {
+                                          int beam, double* cell_acc = nullptr) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.L;
   const long V = L.V;
@@ -595,11 +598,27 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int
     constexpr int l = Tab<N>::t.l[C][I];
     t[I] = ((-du[I]) - Cl[l] * u[I]) / S;
   });
-  gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
   gkd s = rec;
   gkd cl = rec + ne;
   const double w = a.L.dvol;
-  for (int e0 = 0; e0 < ne; e0 += kBatch) {
+  if constexpr (CELL) {
+    static_assert(kCellE <= kBatch, "one batch of elements");
+#pragma unroll
+    for (int e = 0; e < kCellE; ++e) {
+      double acc = 0.0;
+      if (e < ne) {
+        const double se = s[e];
+        static_for<0, NC>([&](auto Ic) {
+          constexpr int I = decltype(Ic)::value;
+          constexpr int l = Tab<N>::t.l[C][I];
+          acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
+        });
+      }
+      cell_acc[e] = acc;
+    }
+  }
+  gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
+  for (int e0 = 0; e0 < (CELL ? 0 : ne); e0 += kBatch) {
     double acc[kBatch];
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) acc[b] = stream_load<DIM>(rb + batch_index(e0, b, ne) * V + p);
@@ -638,16 +657,16 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int
   }
 }
 
-template <int N, int DIM, int C>
+template <int N, int DIM, int C, bool CELL = false>
 __device__ __forceinline__ void dens_class(const DensArgs& a, gkd rec, double int_fac,
                                            gcd __restrict__ u_b, gcd __restrict__ lam_b, int i, int j, int k, long p,
-                                           int beam) {
+                                           int beam, double* cell_acc = nullptr) {
   constexpr int NC = ncls<N, C>();
   gcd __restrict__ lc = lam_b + a.L.cls_off[C];
   double lam[NC];
 #pragma unroll
   for (int m = 0; m < NC; ++m) lam[m] = lc[m * a.L.V + p];
-  dens_core<N, DIM, C>(a, rec, int_fac, u_b, lam, i, j, k, p, beam);
+  dens_core<N, DIM, C, CELL>(a, rec, int_fac, u_b, lam, i, j, k, p, beam, cell_acc);
 }
 
 // Fused reverse pass of step i (reference adjoint, pn_intensities.jl:391-401):
@@ -662,9 +681,9 @@ __device__ __forceinline__ void dens_class(const DensArgs& a, gkd rec, double in
 //   a.rec3: adjoint record of step i + 1 (null for i = n)
 //   a.a1:   run the first odd half step of step i (0 for the final pullback at u_0)
 //   a.dens: u = u_i, rec = density record of step i, int_fac
-template <int N, int DIM, int C>
+template <int N, int DIM, int C, bool CELL = false>
 __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam_b, gcd __restrict__ u_b, int i, int j,
-                                          int k, long p, int beam) {
+                                          int k, long p, int beam, double* cell_acc = nullptr) {
   constexpr int NC = ncls<N, C>();
   const Layout& L = a.adj.L;
   const long V = L.V;
@@ -699,7 +718,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
       }
 #pragma unroll
       for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, l1[m]);
-      dens_core<N, DIM, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
+      dens_core<N, DIM, C, CELL>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam, cell_acc);
       return;
     }
   }
@@ -708,7 +727,7 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
 #pragma unroll
     for (int m = 0; m < NC; ++m) l0[m] = l1[m];
   }
-  dens_core<N, DIM, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam);
+  dens_core<N, DIM, C, CELL>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, l0, i, j, k, p, beam, cell_acc);
   if (a.a1) {
     update_core<N, DIM, C, kAdjoint>(a.adj, cst(a.adj.rec), a.adj.dt, lam_b, l0, l1, i, j, k, p, beam);
   } else {
@@ -773,7 +792,7 @@ __device__ __forceinline__ void class_pairs_2d(const Layout& L, int pair, F&& f)
       const int j = j0 + sub * SUB_ROWS;
       if (i >= L.Mx || j >= L.My) return;
       const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0));
-      if (interior) f(Cc, i, j, i + (long)L.X * j);
+      if (interior) f(Cc, i, j, i + (long)L.X * j, sub - s0);
     }
   };
   if ((wave & 1) == 0) {
@@ -785,19 +804,78 @@ __device__ __forceinline__ void class_pairs_2d(const Layout& L, int pair, F&& f)
   }
 }
 
-template <int N>
-__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int pair) {
+// With a.dens.rb_cell each job returns its class's density gradient per
+// element; the wavefronts leave them in LDS (slot C / 2: eee, eoe, oee, ooe),
+// and after a block barrier the first two wavefronts add the four classes of
+// each cell, in that order, to the cell's rho_bar with one read and write.
+template <int N, bool CELL>
+__device__ __forceinline__ void rev_body_cw_impl(const RevArgs& a, int beam, int pair) {
   const Layout& L = a.adj.L;
   const long bo = (long)beam * L.beam_stride;
   gcd lam_b = glb(a.adj.from) + bo;
   gcd u_b = glb(a.dens.u) + bo;
-  class_pairs_2d<N>(L, pair, [&](auto Cc, int i, int j, long p) {
-    constexpr int C = decltype(Cc)::value;
-    if constexpr (parity_of(C) == 1)
-      rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
-    else
-      dens_class<N, 2, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam);
-  });
+  if constexpr (!CELL) {
+    class_pairs_2d<N>(L, pair, [&](auto Cc, int i, int j, long p, int) {
+      constexpr int C = decltype(Cc)::value;
+      if constexpr (parity_of(C) == 1)
+        rev_class<N, 2, C>(a, lam_b, u_b, i, j, 0, p, beam);
+      else
+        dens_class<N, 2, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam);
+    });
+  } else {
+    __shared__ double part[4][2][kCellE][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this wavefront's two slots start at zero (jobs outside the grid or on
+    // ghost points add nothing)
+    const int s_odd = wave >> 1, c_odd = (wave & 1) ? 2 : 1, c_even = (wave & 1) ? 3 : 0;
+#pragma unroll
+    for (int e = 0; e < kCellE; ++e) {
+      part[c_odd][s_odd][e][lane] = 0.0;
+      part[c_even][1 - s_odd][e][lane] = 0.0;
+    }
+    class_pairs_2d<N>(L, pair, [&](auto Cc, int i, int j, long p, int sub) {
+      constexpr int C = decltype(Cc)::value;
+      double acc[kCellE];
+      if constexpr (parity_of(C) == 1)
+        rev_class<N, 2, C, true>(a, lam_b, u_b, i, j, 0, p, beam, acc);
+      else
+        dens_class<N, 2, C, true>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, 0, p, beam, acc);
+#pragma unroll
+      for (int e = 0; e < kCellE; ++e) part[C / 2][sub][e][lane] = acc[e];
+    });
+    __syncthreads();
+    if (threadIdx.x < 128) {
+      constexpr int SUB_ROWS = 64 / kCwTileX;
+      const int sub = threadIdx.x >> 6;
+      const int tile = pair >> 1, s = 2 * (pair & 1) + sub;
+      const int i = (tile % L.cw_tiles_x) * kCwTileX + lane % kCwTileX;
+      const int j = (tile / L.cw_tiles_x) * kCwTileX + s * SUB_ROWS + lane / kCwTileX;
+      if (i < L.Mx && j < L.My) {
+        const long p = i + (long)L.X * j;
+        gd __restrict__ rb = glb(a.dens.rho_bar) + beam * a.dens.rb_stride;
+        const int ne = L.n_e;
+#pragma unroll
+        for (int e = 0; e < kCellE; ++e) {
+          if (e < ne) {
+            double v = stream_load<2>(rb + e * L.V + p);
+            v += part[0][sub][e][lane];
+            v += part[1][sub][e][lane];
+            v += part[2][sub][e][lane];
+            v += part[3][sub][e][lane];
+            stream_store<2>(rb + e * L.V + p, v);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void rev_body_cw(const RevArgs& a, int beam, int pair) {
+  if (a.dens.rb_cell)
+    rev_body_cw_impl<N, true>(a, beam, pair);
+  else
+    rev_body_cw_impl<N, false>(a, beam, pair);
 }
 
 // Unbounded, the fused reverse pass at P7 lands two registers above the

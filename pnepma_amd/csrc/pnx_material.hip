@@ -64,9 +64,11 @@ __global__ void densities_kernel(MatGeom g, const double* __restrict__ w, const 
 // With box_rb_in_grad, grad holds on entry the density gradient that the
 // absorption pullback gathered per box (absorb_pullback_kernel), whose
 // MassFraction pullback is added last.
+// With rb_cell, rho_bar holds one gradient per cell ([b][e][V] at the cell's
+// eee index), already summed over the cell's class points.
 __global__ void pullback_kernel(MatGeom g, const double* __restrict__ w, const double* __restrict__ rho_pure,
                                 const double* __restrict__ rho_bar, long rb_stride, int n_beams,
-                                const double* __restrict__ extra_eee, int box_rb_in_grad, double* grad) {
+                                const double* __restrict__ extra_eee, int box_rb_in_grad, int rb_cell, double* grad) {
   const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long n_cells = (long)g.M[0] * g.M[1] * g.M[2];
   if (cell >= n_cells) return;
@@ -81,13 +83,14 @@ __global__ void pullback_kernel(MatGeom g, const double* __restrict__ w, const d
     R += wc[e] / rho_pure[e];
     acc[e] = 0.0;
   }
-  for (int ci = 0; ci < g.n_cls; ++ci) {
+  for (int ci = 0; ci < (rb_cell ? 1 : g.n_cls); ++ci) {
     const ClassGeom& c = g.cls[ci];
     const int cc[3] = {cx, cy, cz};
     int lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
       lo[a] = cc[a];
-      hi[a] = (c.odd[a] && cc[a] == g.M[a] - 1) ? g.M[a] : cc[a];  // odd axis: the last box also has point M
+      // odd axis: the last box also has point M (a per-cell gradient sits at the eee index)
+      hi[a] = (!rb_cell && c.odd[a] && cc[a] == g.M[a] - 1) ? g.M[a] : cc[a];
       if (c.n[a] == 1) lo[a] = hi[a] = 0;
     }
     for (int k = lo[2]; k <= hi[2]; ++k)
@@ -98,7 +101,7 @@ __global__ void pullback_kernel(MatGeom g, const double* __restrict__ w, const d
           double dot = 0.0;
           for (int e = 0; e < ne; ++e) {
             double s = 0.0;
-            for (int b = 0; b < n_beams; ++b) s += rho_bar[b * rb_stride + c.off + e * g.V + p];
+            for (int b = 0; b < n_beams; ++b) s += rho_bar[b * rb_stride + (rb_cell ? 0 : c.off) + e * g.V + p];
             if (g.cls_id[ci] == 0 && extra_eee) s += extra_eee[e * g.V + p];
             rb[e] = s;
             dot += wc[e] * s;
@@ -384,12 +387,12 @@ hipError_t launch_material_densities(const Layout& L, const int class_n[8][3], c
 hipError_t launch_material_pullback(const Layout& L, const int class_n[8][3], const long rho_off[8],
                                     const int ncls_of[8], const double* w, const double* rho_pure,
                                     const double* rho_bar, long rb_stride, int n_beams, const double* extra_eee,
-                                    bool box_rb_in_grad, double* grad, hipStream_t s) {
+                                    bool box_rb_in_grad, double* grad, hipStream_t s, bool rb_cell) {
   if (L.n_e > kMaxE) return hipErrorInvalidValue;
   MatGeom g = make_mat_geom(L, class_n, rho_off, class_active_dims, ncls_of);
   const long n_cells = (long)L.Mx * L.My * L.Mz;
   hipLaunchKernelGGL(pullback_kernel, dim3((unsigned)((n_cells + 255) / 256)), dim3(256), 0, s, g, w, rho_pure,
-                     rho_bar, rb_stride, n_beams, extra_eee, box_rb_in_grad ? 1 : 0, grad);
+                     rho_bar, rb_stride, n_beams, extra_eee, box_rb_in_grad ? 1 : 0, rb_cell ? 1 : 0, grad);
   return hipGetLastError();
 }
 

@@ -288,6 +288,10 @@ struct Request {
   double* pinned_out = nullptr;  // results land here (inside `pinned`), then go to the caller's arrays
   // given as cell fractions: the material and absorption are set on the device at admission
   bool spec = false;
+  // 2D cell-fraction request whose caller takes no per-class density
+  // gradient: its reverse passes add the classes of a cell into one per-cell
+  // rho_bar (DensArgs::rb_cell), which only the material pullback reads
+  bool rb_cell = false;
   long abs_pts[3] = {}, abs_edges[3] = {}, abs_mac = 0, abs_am = 0, abs_emb = 0, abs_rp = 0, abs_el = 0;
   double inv_span[3] = {}, detector[3] = {}, beam_x0 = 0.0;
   AbsorptionGeometry ag{};
@@ -473,6 +477,7 @@ DensArgs make_dens(pnx_server* s, const Request& r, int i, const double* u, doub
   a.rho = r.rho;
   density_read_offsets(r.g, r.spec, a.rho_off);
   for (int c = 0; c < 8; ++c) a.rb_off[c] = r.g.rho_off[c];
+  a.rb_cell = r.rb_cell ? 1 : 0;
   a.coef = coef_for(s, r.d.N, false);
   a.rec = r.rec_d + (long)i * r.g.rec_len;
   a.prof = r.prof;
@@ -751,7 +756,8 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
   for (Request* r : plan.copy_out) {
     const long nI = r->d.n_lines, nrb = r->sz.rb, ngen = r->sz.gen;
     if (hipMemcpyAsync(r->pinned_out, r->I_dev, nI * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(r->pinned_out + nI, r->rb, nrb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (!r->rb_cell &&
+         hipMemcpyAsync(r->pinned_out + nI, r->rb, nrb * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipMemcpyAsync(r->pinned_out + nI + nrb, r->gen, ngen * sizeof(double), hipMemcpyDeviceToHost, st) !=
             hipSuccess)
       return -1;
@@ -768,7 +774,7 @@ int launch_plan(Lane& ln, TickPlan& plan, int buf) {
           launch_absorption_pullback(r->ag, g.L, r->abs + r->abs_mac, r->ac, r->rho + g.rho_off[0], am, el, r->gen,
                                      gs, 1, r->grad, st) != hipSuccess ||
           launch_material_pullback(g.L, g.class_n, g.rho_off, g.T.n, r->cell_w, r->abs + r->abs_rp, r->rb,
-                                   g.rho_total, 1, r->extra, true, r->grad, st) != hipSuccess ||
+                                   g.rho_total, 1, r->extra, true, r->grad, st, r->rb_cell) != hipSuccess ||
           hipMemcpyAsync(r->pinned_out + nI + nrb + ngen, r->grad, r->sz.grad * sizeof(double),
                          hipMemcpyDeviceToHost, st) != hipSuccess)
         return -1;
@@ -1260,7 +1266,12 @@ int pnx_server_submit_spec(pnx_server* s, const pnx_request_spec* q, int64_t* ti
   r->beam_x0 = q->beam_position[0];
   r->pinned_out = h + o.in_end;
   r->I_out = q->I_out;
-  for (int c = 0; c < 8; ++c) r->rho_bar[c] = q->rho_bar[c];
+  bool any_rb = false;
+  for (int c = 0; c < 8; ++c) {
+    r->rho_bar[c] = q->rho_bar[c];
+    any_rb |= q->rho_bar[c] != nullptr;
+  }
+  r->rb_cell = !any_rb && g.L.dims == 2 && ne <= kCellE;
   r->gen_out = q->gen_out;
   r->grad_out = q->grad_out;
   r->timing.arrival_s = q->arrival_s;

@@ -11,7 +11,7 @@ from pnepma_amd import serving
 from pnepma_amd.problem import solve_inputs
 from pnepma_amd.solver import DeviceProblem
 
-from .helpers import rel_err
+from .helpers import assert_physical, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -197,3 +197,38 @@ def test_spec_requests_match_prepared_requests_and_the_oracle(oracle_lib):
         print(f"N={p.N} M={p.gspec.M}: spec vs prepared {max(e):.2e}, gradient vs device problem {e_g:.2e}, "
               f"vs oracle {e_o:.2e}, I vs oracle {rel_err(rq.I_out, I_ref):.2e}", flush=True)
         assert max(e) < 1e-12 and e_g < 1e-11 and e_o < 1e-8 and rel_err(rq.I_out, I_ref) < 1e-10
+
+
+@pytest.mark.parametrize("taped", [True, False], ids=["taped", "checkpointed"])
+def test_spec_requests_per_cell_density_gradient(oracle_lib, taped):
+    """2D requests given as cell fractions whose caller takes no per-class
+    density gradient: their reverse passes add the four classes of a cell into
+    one per-cell rho_bar in the block (DensArgs::rb_cell), which only the
+    material pullback reads.  Intensities equal those of the same requests
+    returning per-class gradients bit for bit (the forward is the same), the
+    material gradients agree to rounding, and both agree with the oracle's."""
+    specs = [s for s in serving.serving_requests(24, seed=29, sizes_1d=(30, 70), sizes_2d=(10, 40), max_steps=40)
+             if s.problem.gspec.axes[1].M > 1][:6]
+    assert len(specs) >= 4 and all(len(s.material.elements) <= 4 for s in specs)
+    per_class = [serving.SpecRequest(s, want_gradients=True) for s in specs]
+    per_cell = [serving.SpecRequest(s) for s in specs]
+    srv = serving.Server(hbm_budget_bytes=1 << 30, max_active=64)
+    if not taped:
+        srv.set_tape_limit(0)
+    tk = [srv.submit_spec(rq, arrival_s=0.001 * n) for n, rq in enumerate(per_class + per_cell)]
+    for t in tk:
+        assert srv.wait(t).status == 0
+    srv.close()
+    for spec, a, b in zip(specs, per_class, per_cell):
+        p = spec.problem
+        pr = serving.prepare(spec, want_rho_bar=True)
+        I_ref, rb, acb, noab = oracle_lib.forward_adjoint(p, pr.si, spec.I_bar)
+        mac = np.array([[m for m in row] for row in a.mac])
+        g_ref = oracle_lib.material_gradient(p, spec.material, spec.detector, mac, rb, acb, noab)
+        e_cells = rel_err(b.material_gradient(), a.material_gradient())
+        e_o = rel_err(b.material_gradient(), g_ref)
+        print(f"N={p.N} M={p.gspec.M} ne={len(spec.material.elements)}: per-cell vs per-class gradient {e_cells:.2e}, "
+              f"vs oracle {e_o:.2e}", flush=True)
+        assert np.array_equal(a.I_out, b.I_out)
+        assert_physical(b.I_out)
+        assert e_cells < 1e-12 and e_o < 1e-8
