@@ -889,50 +889,71 @@ __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(Re
   rev_body_cw<N>(a, beam, pair);
 }
 
-// 3D form of the fused reverse pass with one class per wavefront: two blocks
-// take the same 64 cells of one z-plane (sub-tile item & 3 of cw tile
-// item >> 2, cw tiles x-fastest, then planes), the first the classes 0-3 and
-// the second 4-7, wavefront w of block half h the class 4h + w; block item2
-// is (item, half) = (item2 >> 1, item2 & 1), 2 n_cw_items blocks per beam.
-// One thread per cell over all eight classes needed 179 VGPRs at P5
-// (occupancy 2).
+// 3D form of the fused reverse pass with one class per wavefront.  Sub-tiles
+// are the 64-cell sub-tiles of the 2D form in every z-plane (sub-tile
+// item & 3 of cw tile item >> 2, cw tiles x-fastest, then planes).  Block
+// item2 takes half h = item2 & 1 of the classes (h = 0: eee, eeo, eoe, eoo;
+// h = 1: oee, oeo, ooe, ooo; each half two odd and two even classes) at the
+// two sub-tiles 2 (q & 1) and 2 (q & 1) + 1 of cw tile and plane q >> 1,
+// q = item2 >> 1: n_cw_items blocks per beam.  As in 2D, wavefront w takes one
+// odd-class job at one sub-tile and one even-class job at the other, so the
+// four wavefronts finish together.  One thread per cell over all eight classes
+// needed 179 VGPRs at P5 (occupancy 2).
 template <int N>
 __device__ __forceinline__ void rev_body_cw3(const RevArgs& a, int beam, int item2) {
   const Layout& L = a.adj.L;
-  const int half = item2 & 1, item = item2 >> 1;
+  const int half = item2 & 1, q = item2 >> 1;
   constexpr int SUB_ROWS = 64 / kCwTileX;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cw_tiles_y = (L.My + kCwTileX - 1) / kCwTileX;
-  const int sub = item & 3, r = item >> 2;
+  const int s0 = 2 * (q & 1), r = q >> 1;
   const int tile = r % (L.cw_tiles_x * cw_tiles_y), k = r / (L.cw_tiles_x * cw_tiles_y);
   const int tx = tile % L.cw_tiles_x, ty = tile / L.cw_tiles_x;
   const int i = tx * kCwTileX + lane % kCwTileX;
-  const int j = ty * kCwTileX + sub * SUB_ROWS + lane / kCwTileX;
-  if (i >= L.Mx || j >= L.My || k >= L.Mz) return;
-  const long p = i + (long)L.X * (j + (long)L.Y * k);
+  const int j0 = ty * kCwTileX + lane / kCwTileX;
+  const int s_odd = s0 + (wave >> 1), s_even = s0 + 1 - (wave >> 1);
   const long bo = (long)beam * L.beam_stride;
   gcd lam_b = glb(a.adj.from) + bo;
   gcd u_b = glb(a.dens.u) + bo;
-  static_for<0, 8>([&](auto Cc) {
+  auto job = [&](auto Cc, int sub) {
     constexpr int C = decltype(Cc)::value;
     if constexpr (ncls<N, C>() > 0) {
-      if (4 * half + wave == C) {
-        const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0) || (odd_in(C, 2) && k == 0));
-        if (interior) {
-          if constexpr (parity_of(C) == 1)
-            rev_class<N, 3, C>(a, lam_b, u_b, i, j, k, p, beam);
-          else
-            dens_class<N, 3, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
-        }
-      }
+      const int j = j0 + sub * SUB_ROWS;
+      if (i >= L.Mx || j >= L.My || k >= L.Mz) return;
+      const bool interior = !((odd_in(C, 0) && i == 0) || (odd_in(C, 1) && j == 0) || (odd_in(C, 2) && k == 0));
+      if (!interior) return;
+      const long p = i + (long)L.X * (j + (long)L.Y * k);
+      if constexpr (parity_of(C) == 1)
+        rev_class<N, 3, C>(a, lam_b, u_b, i, j, k, p, beam);
+      else
+        dens_class<N, 3, C>(a.dens, cst(a.dens.rec), a.dens.int_fac, u_b, lam_b, i, j, k, p, beam);
     }
-  });
+  };
+  using std::integral_constant;
+  // odd classes: eeo (1), eoe (2) | oee (4), ooo (7); even: eee (0), eoo (3) | oeo (5), ooe (6)
+  if (half == 0) {
+    if ((wave & 1) == 0) {
+      job(integral_constant<int, 1>{}, s_odd);
+      job(integral_constant<int, 0>{}, s_even);
+    } else {
+      job(integral_constant<int, 2>{}, s_odd);
+      job(integral_constant<int, 3>{}, s_even);
+    }
+  } else {
+    if ((wave & 1) == 0) {
+      job(integral_constant<int, 4>{}, s_odd);
+      job(integral_constant<int, 5>{}, s_even);
+    } else {
+      job(integral_constant<int, 7>{}, s_odd);
+      job(integral_constant<int, 6>{}, s_even);
+    }
+  }
 }
 
 template <int N>
 __global__ __launch_bounds__(kBlock) void rev_kernel_cw3(RevArgs a) {
   int beam, item2;
-  block_coords(2 * a.adj.L.n_cw_items, a.adj.n_beams, beam, item2);
+  block_coords(a.adj.L.n_cw_items, a.adj.n_beams, beam, item2);
   rev_body_cw3<N>(a, beam, item2);
 }
 
@@ -962,9 +983,13 @@ __global__ __launch_bounds__(kBlock, DIM == 2 ? rev_cw_min_blocks(N) : 1) void p
 
 // The fused odd pair has its own launch: in one kernel with the two kinds of
 // phase B the compiler allocated more registers than any of them needs (2D
-// P7: 209 VGPRs, P9: 255 and occupancy 1; bounded, it spilled).
+// P7: 209 VGPRs, P9: 255 and occupancy 1; bounded, it spilled).  At 2D P9 it
+// needs 175 VGPRs, 7 over three waves per SIMD; bounded to three it keeps 36
+// bytes per lane in scratch and the P9 lane serves 0.8 % faster.
+constexpr int pair_min_blocks(int N, int DIM) { return N == 9 && DIM == 2 ? 3 : 1; }
+
 template <int N, int DIM>
-__global__ __launch_bounds__(kBlock) void pair_grouped(const HalfStepArgs* __restrict__ half,
+__global__ __launch_bounds__(kBlock, pair_min_blocks(N, DIM)) void pair_grouped(const HalfStepArgs* __restrict__ half,
                                                        const int2* __restrict__ items,
                                                        const int* __restrict__ n_items) {
   const int it = grouped_first();
@@ -1276,7 +1301,7 @@ hipError_t rev_order(int dims, const RevArgs& a, hipStream_t s) {
     case 1: hipLaunchKernelGGL((rev_kernel<N, 1>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((rev_kernel_cw<N>), dim3(a.adj.L.n_cw_items / 2 * a.adj.n_beams), block, 0, s, a); break;
     case 3:
-      hipLaunchKernelGGL((rev_kernel_cw3<N>), dim3(2 * a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a);
+      hipLaunchKernelGGL((rev_kernel_cw3<N>), dim3(a.adj.L.n_cw_items * a.adj.n_beams), block, 0, s, a);
       break;
     default: return hipErrorInvalidValue;
   }

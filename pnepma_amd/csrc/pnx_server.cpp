@@ -558,10 +558,10 @@ void add_half(PhaseWork& w, const HalfStepArgs& a, int kind) {
 void add_rev(PhaseWork& w, const RevArgs& a) {
   const int idx = (int)w.rev.size();
   w.rev.push_back(a);
-  // 2D: one class per wavefront, one block per two 64-cell sub-tiles of a cw
-  // tile; 3D: two blocks per sub-tile (classes 0-3 and 4-7)
+  // one class per wavefront, 2D: one block per two 64-cell sub-tiles of a
+  // cw tile; 3D: two blocks per two sub-tiles (classes 0-3 and 4-7)
   const int dims = a.adj.L.dims;
-  const int n = dims == 2 ? a.adj.L.n_cw_items / 2 : (dims == 3 ? 2 * a.adj.L.n_cw_items : a.adj.L.n_tiles);
+  const int n = dims == 2 ? a.adj.L.n_cw_items / 2 : (dims == 3 ? a.adj.L.n_cw_items : a.adj.L.n_tiles);
   for (int t = 0; t < n; ++t) w.items.push_back(make_int2(kItemRev << kItemKindShift | idx, t));
 }
 
