@@ -616,22 +616,31 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int
   });
   gkd s = rec;
   gkd cl = rec + ne;
-  const double w = a.L.dvol;
+  // rho_bar_e += dvol dE int_fac sum_I lam_I (s_e t_I + c_{e,l(I)} u_I)
+  //            = wfac (s_e X + sum_l c_{e,l} Y_l), with X = sum_I lam_I t_I and
+  //              Y_l = sum of lam_I u_I over the moments I of degree l:
+  // the sums over moments are formed once for all elements.
+  double X = 0.0, Y[N + 1];
+  static_for<0, N + 1>([&](auto lc) { Y[decltype(lc)::value] = 0.0; });
+  static_for<0, NC>([&](auto Ic) {
+    constexpr int I = decltype(Ic)::value;
+    constexpr int l = Tab<N>::t.l[C][I];
+    X += lam[I] * t[I];
+    Y[l] += lam[I] * u[I];
+  });
+  const double wfac = a.L.dvol * a.dE * int_fac;
+  auto grad_e = [&](int e) {
+    double v = s[e] * X;
+    static_for<0, N + 1>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      if constexpr (class_has_degree<N, C>(l)) v += cl[e * (N + 1) + l] * Y[l];
+    });
+    return v * wfac;
+  };
   if constexpr (CELL) {
     static_assert(kCellE <= kBatch, "one batch of elements");
 #pragma unroll
-    for (int e = 0; e < kCellE; ++e) {
-      double acc = 0.0;
-      if (e < ne) {
-        const double se = s[e];
-        static_for<0, NC>([&](auto Ic) {
-          constexpr int I = decltype(Ic)::value;
-          constexpr int l = Tab<N>::t.l[C][I];
-          acc += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
-        });
-      }
-      cell_acc[e] = acc;
-    }
+    for (int e = 0; e < kCellE; ++e) cell_acc[e] = e < ne ? grad_e(e) : 0.0;
   }
   gd __restrict__ rb = glb(a.rho_bar) + beam * a.rb_stride + a.rb_off[C];
   for (int e0 = 0; e0 < (CELL ? 0 : ne); e0 += kBatch) {
@@ -641,15 +650,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) {
       const int e = e0 + b;
-      if (e < ne) {
-        const double se = s[e];
-        static_for<0, NC>([&](auto Ic) {
-          constexpr int I = decltype(Ic)::value;
-          constexpr int l = Tab<N>::t.l[C][I];
-          acc[b] += lam[I] * (se * t[I] + cl[e * (N + 1) + l] * u[I]) * w * a.dE * int_fac;
-        });
-        stream_store<DIM>(rb + e * V + p, acc[b]);
-      }
+      if (e < ne) stream_store<DIM>(rb + e * V + p, acc[b] + grad_e(e));
     }
   }
   if constexpr (C == 0) {
@@ -665,7 +666,7 @@ __device__ __forceinline__ void dens_core(const DensArgs& a, gkd rec, double int
       for (int b = 0; b < kBatch; ++b) {
         const int r = r0 + b;
         if (r < nl) {
-          const double integral = a.dE * w * sig[r] * int_fac * ib[r];
+          const double integral = a.dE * a.L.dvol * sig[r] * int_fac * ib[r];
           stream_store<DIM>(g + r * V + p, gv[b] + u[0] * integral);
         }
       }
