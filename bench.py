@@ -1,6 +1,6 @@
 """Benchmark of the PN-CSD forward+adjoint hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serve|raster2d_grad|raster3d_step|kernel]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload serve|raster2d_grad|raster3d_step|raster3d_full|kernel]
 
 serve (default): the mixed serving stream -- R = 4096 forward+gradient requests
 (PN order 3-9, 1D 100-400 cells or 2D 64^2-192^2, 10-25 keV, 2-4 elements,
@@ -21,6 +21,7 @@ cores.
 
 raster2d_grad: 2D P7 128x96, 32 beams, forward+adjoint of the batch per step.
 raster3d_step: 3D P5 128x128x96, one beam, a sample of its energy steps.
+raster3d_full: the same workload whole: 64 beams, every energy step (minutes).
 
 Every mode also measures the roofline figure of the dominant kernel (the fused
 even-parity half step of the 2D P7 128x96 batch) with HIP events on the
@@ -602,6 +603,75 @@ def main_raster3d(args):
     print(json.dumps(out), flush=True)
 
 
+def main_raster3d_full(args):
+    """The whole 3D workload: P5 128x128x96 Al/Si inclusion, the 8 x 8 beam
+    raster, all CFL-0.5 energy steps from 25 to 1 keV, forward + reference
+    adjoint with checkpoints every 32, material and absorption set on the
+    device from the cell fractions, the material gradient of every beam summed.
+    Beams run in batches that fit the HBM; about five minutes of GPU time, so it
+    is not part of the default run."""
+    import copy
+    import numpy as np
+    import torch
+    from pnepma_amd import workloads, serving
+    from pnepma_amd.solver import DeviceProblem
+    assert torch.cuda.is_available()
+    torch.cuda.init()
+    p0, mat, stds, det = workloads.inclusion3d(n_steps=8)
+    n_cfl = serving.cfl_steps(p0.N, p0.gspec, mat, p0.elements, p0.E_initial, p0.E_cutoff)
+    p, mat, stds, det = workloads.inclusion3d(n_steps=n_cfl, n_beams=64)
+
+    def beams(idx):
+        q = copy.copy(p)
+        q._cache = {}
+        q.beam_positions = [p.beam_positions[b] for b in idx]
+        return q
+
+    ib = np.ones(len(p.x_rays))
+    probe = DeviceProblem(beams([0]), None, ckpt_every=32)
+    probe.reserve_adjoint()
+    per_beam = probe.device_bytes
+    probe.close()
+    free, total = torch.cuda.mem_get_info()
+    batch = max(1, min(16, int(0.85 * free // per_beam)))
+    progress(f"3D P5 {p.gspec.M}: {n_cfl} steps, 64 beams, checkpoints every 32; "
+             f"{per_beam / 1e9:.1f} GB per beam, batches of {batch}")
+    I_all, grad, t_gpu = [], None, 0.0
+    t0 = time.perf_counter()
+    for b0 in range(0, 64, batch):
+        idx = list(range(b0, min(64, b0 + batch)))
+        dp = DeviceProblem(beams(idx), None, ckpt_every=32)
+        dp.set_cell_fractions(mat, None)
+        dp.set_absorption(mat, det)
+        dp.reserve_adjoint()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        I, _, _ = dp.forward_adjoint(ib, want_rho_bar=False)
+        g = dp.material_gradient()
+        t_gpu += time.perf_counter() - t
+        dp.close()
+        I_all.append(I)
+        grad = g if grad is None else grad + g
+        progress(f"beams {idx[0]}-{idx[-1]}: {time.perf_counter() - t:.1f} s")
+    wall = time.perf_counter() - t0
+    I_all = np.concatenate(I_all)
+    ok = bool(np.all(np.isfinite(I_all)) and np.all(np.isfinite(grad)) and
+              all(np.all(I_all[:, r] > 0) or np.all(I_all[:, r] < 0) for r in range(I_all.shape[1])))
+    out = {
+        "metric": METRIC, "value": 64 / t_gpu, "unit": "beams/s (forward+adjoint of the whole energy range)",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * t_gpu, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D P5 128x128x96, Al/Si with an ellipsoidal inclusion, 8x8 beam raster, "
+                               f"all {n_cfl} CFL-0.5 energy steps of 25 -> 1 keV, checkpoints every 32, "
+                               "material gradient summed over the beams"},
+        "solve_s": t_gpu, "wall_s_with_setup": wall, "beams_per_batch": batch, "device_bytes_per_beam": per_beam,
+        "intensities_finite_one_sign": ok,
+        "intensity_range": [float(np.abs(I_all).min()), float(np.abs(I_all).max())],
+        "gradient_norm": float(np.linalg.norm(grad)),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main_kernel(args):
     """Only the dominant kernel (and a plain 256 MiB device copy for counter
     calibration): used under rocprofv3 to read per-dispatch HBM counters."""
@@ -626,7 +696,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="serve", choices=["serve", "raster2d_grad", "raster3d_step", "kernel"])
+    ap.add_argument("--workload", default="serve",
+                    choices=["serve", "raster2d_grad", "raster3d_step", "raster3d_full", "kernel"])
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grids", default="all", choices=["all", "1d", "2d"],
@@ -644,6 +715,8 @@ def main():
         main_kernel(args)
     elif args.workload == "raster3d_step":
         main_raster3d(args)
+    elif args.workload == "raster3d_full":
+        main_raster3d_full(args)
     else:
         main_raster(args)
 
