@@ -167,3 +167,22 @@ def test_literal_floor_differs_only_on_interior_edges():
                 assert edge, x
     print(f"{differ} of {on_edge} points on interior edges change box under the floating-point floor", flush=True)
     assert 0 < differ < on_edge
+
+
+@pytest.mark.parametrize("M", [(30, 1, 1), (12, 9, 1), (6, 5, 4)], ids=["1d", "2d", "3d"])
+def test_cell_box_densities_equal_the_eee_densities(M):
+    """On a material with one box per grid cell, every class's densities at the
+    points the kernels update (odd axes: indices 1..M-1, even axes 0..M-1) are
+    the eee densities at the same index: odd point i lies on the lower edge of
+    box i, which the oracle assigns to box i.  The kernels of cell-fraction
+    problems read the eee array for every class on this ground."""
+    g, mat, _ = _material(M=M, seed=7)
+    eee = oracle.component_densities(mat, g.class_points(0)).reshape(M[2], M[1], M[0], -1)
+    for c in range(8):
+        if not g.class_active(c):
+            continue
+        n = g.class_size(c)
+        rho = oracle.component_densities(mat, g.class_points(c)).reshape(n[2], n[1], n[0], -1)
+        lo = [1 if (n[a] == M[a] + 1) else 0 for a in range(3)]  # odd axes skip the ghost at 0
+        upd = rho[lo[2]:M[2], lo[1]:M[1], lo[0]:M[0]]
+        assert np.array_equal(upd, eee[lo[2]:, lo[1]:, lo[0]:]), c
