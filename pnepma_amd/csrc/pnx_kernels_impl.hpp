@@ -417,7 +417,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Intensity sums of one beam over the parts [t0, t1) of the per-step partial
 // sums (integrate_step_and_add!, x_ray_intensities.jl:172-191): thread t sums
 // the steps k = 1 + t, 1 + t + 256, ..., each over the parts, then the block
-// reduces and writes out[r].  Every thread of the block must call it.
+// reduces and writes out[r].  Every thread of the block must call it; in a
+// block wider than kBlock only the first kBlock threads sum, so the result is
+// the same for every block width.
 __device__ __forceinline__ void reduce_intensity_body(const double* __restrict__ partial, long step_stride, int t0,
                                                       int t1, const double* __restrict__ rec, int rec_len,
                                                       int sigma_off, int n_steps, int n_lines, double dE, double dvol,
@@ -426,7 +428,8 @@ __device__ __forceinline__ void reduce_intensity_body(const double* __restrict__
   double acc[kMaxLines];
 #pragma unroll
   for (int r = 0; r < kMaxLines; ++r) acc[r] = 0.0;
-  for (int kk = 1 + threadIdx.x; kk <= n_steps; kk += kBlock) {
+  const int n_sum = threadIdx.x < kBlock ? n_steps : 0;
+  for (int kk = 1 + threadIdx.x; kk <= n_sum; kk += kBlock) {
     gcd __restrict__ ps = glb(partial) + kk * step_stride;
     gcd __restrict__ sig = glb(rec) + (long)kk * rec_len + sigma_off;
     for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
@@ -438,7 +441,7 @@ __device__ __forceinline__ void reduce_intensity_body(const double* __restrict__
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int r = 0; r < n_lines && r < kMaxLines; ++r) {
     const double s = wave_sum(acc[r]);
-    if (lane == 0) red[wave][r] = s;
+    if (lane == 0 && wave < kBlock / 64) red[wave][r] = s;
   }
   __syncthreads();
   if (threadIdx.x < n_lines && threadIdx.x < kMaxLines) {
@@ -471,25 +474,31 @@ __device__ __forceinline__ int grouped_first() {
   return (G & 7) == 0 ? (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3) : blockIdx.x;
 }
 
+// Index of this thread within its tile's kBlock threads.  Grid kernels have
+// one tile per block of kBlock threads; in a WIDE block (the 1D march kernel)
+// each group of kBlock threads takes a tile of its own.
+template <bool WIDE>
+__device__ __forceinline__ int tile_thread() { return WIDE ? (int)threadIdx.x & (kBlock - 1) : (int)threadIdx.x; }
+
 // Cell (i, j, k) of this thread in tile `tile` (tile_x(DIM) x 256/tile_x(DIM)
 // cells of one z-plane); false for threads past the grid's edge.
-template <int DIM>
+template <int DIM, bool WIDE = false>
 __device__ __forceinline__ bool tile_cell(const Layout& L, int tile, int& i, int& j, int& k) {
   constexpr int TX = tile_x(DIM), TY = kBlock / TX;
   const int tx = tile % L.tiles_x;
   const int r = tile / L.tiles_x;
   const int ty = r % L.tiles_y;
   k = r / L.tiles_y;
-  i = tx * TX + (int)threadIdx.x % TX;
-  j = ty * TY + (int)threadIdx.x / TX;
+  i = tx * TX + tile_thread<WIDE>() % TX;
+  j = ty * TY + tile_thread<WIDE>() / TX;
   return i < L.Mx && j < L.My;
 }
 
-template <int N, int DIM, int PAR, int MODE>
+template <int N, int DIM, int PAR, int MODE, bool WIDE = false>
 __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const StepView& v, int beam, int tile) {
   const Layout& L = a.L;
   int i, j, k;
-  const bool valid = tile_cell<DIM>(L, tile, i, j, k);
+  const bool valid = tile_cell<DIM, WIDE>(L, tile, i, j, k);
   if (!valid) i = j = k = 0;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
@@ -510,7 +519,7 @@ __device__ __forceinline__ void half_step_body(const HalfStepArgs& a, const Step
   }
   if constexpr (MODE == kForward && PAR == 0) {
     // one partial sum per wavefront and line (no barrier)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = tile_thread<WIDE>() >> 6;
     gcd __restrict__ Wp = glb(a.W);
     gd __restrict__ out = glb(v.partial) + beam * a.partial_stride + ((long)tile * kWavesPerBlock + wave) * L.n_lines;
     const int nl = L.n_lines < kMaxLines ? L.n_lines : kMaxLines;
@@ -552,11 +561,11 @@ __global__ __launch_bounds__(kBlock) void half_step_grouped(const HalfStepArgs* 
 // Fused pair of odd half steps: the last odd half step of energy step k
 // (record v.rec, step v.dt, intermediate output v.dst or none) and the first
 // of step k+1 (v.rec2, v.dt2, output v.dst2).
-template <int N, int DIM, int MODE>
+template <int N, int DIM, int MODE, bool WIDE = false>
 __device__ __forceinline__ void half_step2_body(const HalfStepArgs& a, const StepView& v, int beam, int tile) {
   const Layout& L = a.L;
   int i, j, k;
-  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
+  if (!tile_cell<DIM, WIDE>(L, tile, i, j, k)) return;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
   gd dst = v.dst ? glb(v.dst) + bo : nullptr;
@@ -755,11 +764,11 @@ __device__ __forceinline__ void rev_class(const RevArgs& a, gcd __restrict__ lam
   for (int m = 0; m < NC; ++m) stream_store<DIM>(out + m * V + p, l1[m]);
 }
 
-template <int N, int DIM>
+template <int N, int DIM, bool WIDE = false>
 __device__ __forceinline__ void rev_body(const RevArgs& a, int beam, int tile) {
   const Layout& L = a.adj.L;
   int i, j, k;
-  if (!tile_cell<DIM>(L, tile, i, j, k)) return;
+  if (!tile_cell<DIM, WIDE>(L, tile, i, j, k)) return;
   const long p = i + (long)L.X * (j + (long)L.Y * k);
   const long bo = (long)beam * L.beam_stride;
   gcd lam_b = glb(a.adj.from) + bo;

@@ -4,8 +4,11 @@
 // workgroup barriers instead of kernel boundaries.  A 1D request has at most
 // a few hundred cells (one or two tiles of 256), so a workgroup holds all of
 // it and a step costs a few barrier round trips rather than seven launches.
-// The device functions are the ones the grid kernels use, in the same order
-// and with the same tiles, so the results are bitwise those of the grid path.
+// A workgroup of 2 x 256 threads runs two tiles of a pass side by side, as the
+// grid kernels' blocks do, instead of one after the other.  The device
+// functions are the ones the grid kernels use, in the same order and with the
+// same tiles, so the results are bitwise those of the grid path for either
+// workgroup width.
 #include <hip/hip_runtime.h>
 
 #include "pnx_kernels_impl.hpp"
@@ -17,12 +20,22 @@ namespace {
 __device__ __forceinline__ void block_copy(double* dst_, const double* src_, long n) {
   gd __restrict__ dst = glb(dst_);
   gcd __restrict__ src = glb(src_);
-  for (long x = threadIdx.x; x < n; x += kBlock) dst[x] = src[x];
+  for (long x = threadIdx.x; x < n; x += blockDim.x) dst[x] = src[x];
+}
+
+// The tiles of a pass: the workgroup's kBlock-thread groups take tiles
+// g, g + G, g + 2G, ... (G groups).  g is the same for the whole wavefront;
+// read from the first lane it stays in a scalar register with the tile
+// arithmetic that follows from it.
+template <class F>
+__device__ __forceinline__ void for_tiles(int n_tiles, F&& f) {
+  const int G = (int)blockDim.x / kBlock;
+  for (int t = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kBlock); t < n_tiles; t += G) f(t);
 }
 
 template <int N, int PAR, int MODE>
 __device__ __forceinline__ void half_pass(const HalfStepArgs& a, const StepView& v) {
-  for (int t = 0; t < a.L.n_tiles; ++t) half_step_body<N, 1, PAR, MODE>(a, v, 0, t);
+  for_tiles(a.L.n_tiles, [&](int t) { half_step_body<N, 1, PAR, MODE, true>(a, v, 0, t); });
   __syncthreads();
 }
 
@@ -40,7 +53,7 @@ __device__ __forceinline__ void rev_pass(const MarchArgs& m, int i, const double
   ra.dens.int_fac = int_fac;
   ra.rec3 = a3 ? m.rec_a + (long)(i + 1) * m.rec_len : nullptr;
   ra.a1 = a1 ? 1 : 0;
-  for (int t = 0; t < ra.adj.L.n_tiles; ++t) rev_body<N, 1>(ra, 0, t);
+  for_tiles(ra.adj.L.n_tiles, [&](int t) { rev_body<N, 1, true>(ra, 0, t); });
   __syncthreads();
 }
 
@@ -74,7 +87,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
         // last odd half step of this step and the first of the next, in one pass
         double* dst2 = march_buffer(m, o.dst2_buf, o.dst2_idx, st);
         const StepView v{dst, dst, dst2 == dst ? nullptr : dst, rec, nullptr, h, rec + m.rec_len, dst2, h};
-        for (int t = 0; t < L.n_tiles; ++t) half_step2_body<N, 1, kForward>(m.fwd, v, 0, t);
+        for_tiles(L.n_tiles, [&](int t) { half_step2_body<N, 1, kForward, true>(m.fwd, v, 0, t); });
         __syncthreads();
       } else {
         half_pass<N, 1, kForward>(m.fwd, StepView{dst, dst, dst, rec, nullptr, h});
@@ -95,9 +108,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& m) {
 // All PN orders in one launch: a tick of the 1D lane then lasts as long as
 // its slowest request, not the sum of one launch per order.  The kernel
 // allocates the registers of the largest order for every workgroup; the bound
-// of two workgroups per CU keeps it at 249 VGPRs without spills (unbounded it
-// took 256 VGPRs + 7 AGPRs, so each request held a whole CU).
-__global__ __launch_bounds__(kBlock, 2) void march_kernel(const MarchArgs* __restrict__ args, int n) {
+// of two waves per SIMD (two workgroups of kBlock threads per CU, or one of
+// kMarchWide) keeps it at 249 VGPRs without spills (unbounded it took 256
+// VGPRs + 7 AGPRs, so each request held a whole CU).
+__global__ __launch_bounds__(kMarchWide, 1) void march_kernel(const MarchArgs* __restrict__ args, int n) {
   for (int r = blockIdx.x; r < n; r += gridDim.x) {
     switch (args[r].N) {
       case 1: march_body<1>(args[r]); break;
@@ -112,9 +126,10 @@ __global__ __launch_bounds__(kBlock, 2) void march_kernel(const MarchArgs* __res
 
 }  // namespace
 
-hipError_t launch_march(const MarchArgs* args, int n_requests, hipStream_t s) {
+hipError_t launch_march(const MarchArgs* args, int n_requests, int threads, hipStream_t s) {
   if (n_requests <= 0) return hipSuccess;
-  const dim3 grid(n_requests < 8192 ? n_requests : 8192), block(kBlock);
+  if (threads != kBlock && threads != kMarchWide) return hipErrorInvalidValue;
+  const dim3 grid(n_requests < 8192 ? n_requests : 8192), block(threads);
   hipLaunchKernelGGL(march_kernel, grid, block, 0, s, args, n_requests);
   return hipGetLastError();
 }

@@ -180,7 +180,11 @@ struct MarchArgs {
   int n_ops;          // operations to run in this launch
 };
 
-// Runs args[0..n_requests) (1D, any PN orders), one workgroup per request.
-hipError_t launch_march(const MarchArgs* args, int n_requests, hipStream_t s);
+// Widest march workgroup: two tiles of kBlock cells side by side.
+constexpr int kMarchWide = 2 * kBlock;
+
+// Runs args[0..n_requests) (1D, any PN orders), one workgroup of `threads`
+// (kBlock or kMarchWide) per request.  The results do not depend on the width.
+hipError_t launch_march(const MarchArgs* args, int n_requests, int threads, hipStream_t s);
 
 }  // namespace pnx

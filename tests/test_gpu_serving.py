@@ -232,3 +232,51 @@ def test_spec_requests_per_cell_density_gradient(oracle_lib, taped):
         assert np.array_equal(a.I_out, b.I_out)
         assert_physical(b.I_out)
         assert e_cells < 1e-12 and e_o < 1e-8
+
+
+def _serve_1d(preps, max_active, together):
+    srv = serving.Server(hbm_budget_bytes=4 << 30, max_active=max_active)
+    # with one arrival time for all, the first tick admits every request, so
+    # each march launch holds all of them
+    at = srv.clock() + 0.3 if together else 0.0
+    for t in [srv.submit(pr, arrival_s=at) for pr in preps]:
+        assert srv.wait(t).status == 0
+    srv.close()
+
+
+def test_served_1d_multi_tile_requests_do_not_depend_on_the_workgroup_width(oracle_lib):
+    """1D requests of two and three tiles (257-600 cells): a march launch of at
+    most 16 requests runs a 512-thread workgroup per request (two tiles of a
+    pass side by side), one of more requests a 256-thread workgroup (tiles in
+    turn).  The results are bitwise the same, equal the request solved alone
+    and match the oracle."""
+    drawn = serving.serving_requests(300, seed=41, sizes_1d=(257, 600), sizes_2d=(10, 12), max_steps=24)
+    specs = [s for s in drawn if s.problem.gspec.dims == 1][:72]
+    assert len(specs) == 72
+    assert {(s.problem.gspec.axes[0].M + 255) // 256 for s in specs} == {2, 3}
+    preps = [serving.prepare(s, want_rho_bar=True) for s in specs]
+    for k in range(0, len(preps), 8):
+        _serve_1d(preps[k:k + 8], 8, together=False)
+    wide = [(pr.I_out.copy(), [None if x is None else x.copy() for x in pr.rho_bar], pr.gen.copy()) for pr in preps]
+    _serve_1d(preps, 128, together=True)
+    for spec, pr, (I, rb, gen) in zip(specs, preps, wide):
+        p = spec.problem
+        assert np.array_equal(pr.I_out, I), (p.N, p.gspec.M)
+        for c in p.gspec.active_classes():
+            assert np.array_equal(pr.rho_bar[c], rb[c]), (p.N, p.gspec.M, c)
+        assert np.array_equal(pr.gen, gen)
+    by_order = {}
+    for spec, pr in zip(specs, preps):
+        by_order.setdefault(spec.problem.N, (spec, pr))
+    for spec, pr in by_order.values():
+        p = spec.problem
+        print(f"N={p.N} M={p.gspec.M} steps={p.n_steps} I={pr.I_out}", flush=True)
+        I, rb, gen = _alone(spec)
+        assert np.array_equal(pr.I_out, I) or rel_err(pr.I_out, I) < 1e-12
+        for c in p.gspec.active_classes():
+            assert rel_err(pr.rho_bar[c], rb[c]) < 1e-12
+        Io, rbo, acb, _ = oracle_lib.forward_adjoint(p, pr.si, spec.I_bar)
+        assert_physical(pr.I_out)
+        assert rel_err(pr.I_out, Io) < 1e-10
+        for c in p.gspec.active_classes():
+            assert rel_err(pr.rho_bar[c], rbo[c]) < 1e-8
