@@ -315,8 +315,17 @@ struct Request {
   int ops_left = 0;            // operations not yet issued
   int64_t done_tick = -1;      // lane tick that copied the results back
   std::atomic<bool> cancel{false};  // set by pnx_server_cancel: issue nothing more
+  // Ticks taken so far, counted by the worker outside the server mutex; poll
+  // and wait report it as timing.ticks.  The other timing fields are written
+  // and read under the mutex.
+  std::atomic<int32_t> ticks{0};
   pnx_timing timing{};
   bool done = false;
+  pnx_timing report() const {
+    pnx_timing t = timing;
+    t.ticks = ticks.load(std::memory_order_relaxed);
+    return t;
+  }
 };
 
 struct GroupKey {
@@ -631,7 +640,7 @@ void plan_grid(pnx_server* s, Request& r, SubTick& plan, TickPlan& whole) {
     default:
       break;
   }
-  r.timing.ticks += 1;
+  r.ticks.fetch_add(1, std::memory_order_relaxed);
 }
 
 // Plan the next slice of operations of request r on the 1D lane.
@@ -643,7 +652,7 @@ void plan_march(pnx_server* s, Request& r, TickPlan& plan) {
   r.ops_left -= m.n_ops;
   plan.march.push_back(m);
   if (r.ops_left == 0) plan.copy_out.push_back(&r);
-  r.timing.ticks += 1;
+  r.ticks.fetch_add(1, std::memory_order_relaxed);
 }
 
 int ensure_pinned(Lane& ln, int b, size_t bytes) {
@@ -1296,7 +1305,7 @@ int pnx_server_poll(pnx_server* s, int64_t ticket, int32_t* done, pnx_timing* t)
   auto it = s->all.find(ticket);
   if (it == s->all.end()) return set_error(PNX_ERR_ARG, "unknown ticket");
   *done = it->second->done ? 1 : 0;
-  if (t) *t = it->second->timing;
+  if (t) *t = it->second->report();
   if (s->error) return set_error(s->error, s->error_msg);
   return PNX_OK;
 }
@@ -1308,7 +1317,7 @@ int pnx_server_wait(pnx_server* s, int64_t ticket, pnx_timing* t) {
   if (it == s->all.end()) return set_error(PNX_ERR_ARG, "unknown ticket");
   auto r = it->second;
   s->done_cv.wait(lk, [&] { return r->done || s->error != 0; });
-  if (t) *t = r->timing;
+  if (t) *t = r->report();
   if (s->error) return set_error(s->error, s->error_msg);
   s->all.erase(ticket);  // the ticket is released: its request is complete
   return PNX_OK;

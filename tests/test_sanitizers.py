@@ -1,8 +1,10 @@
-"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer, on the CPU.
+"""Host code under AddressSanitizer, UndefinedBehaviorSanitizer and
+ThreadSanitizer, on the CPU.
 
-Three programs are built with g++/gcc -fsanitize=address,undefined
--fno-sanitize-recover=all and run; each prints "ok" only if its own checks pass
-and the sanitizers report nothing (a report aborts the program):
+Four programs are built with g++/gcc -fsanitize=address,undefined
+-fno-sanitize-recover=all and run, the last also with clang -fsanitize=thread;
+each prints "ok" only if its own checks pass and the sanitizers report nothing
+(a report aborts the program):
   * tests/host_cpp/test_host_logic.cpp: the server's arena allocator against a
     reference model (including exhaustion and coalescing), the admission
     decision, and the served operation sequence for taped and checkpointed
@@ -10,7 +12,12 @@ and the sanitizers report nothing (a report aborts the program):
   * tests/host_cpp/test_host_setup.cpp: descriptor validation with malformed
     descriptors, the C++ records / beam moments / beam profiles, layout
     packing;
-  * tests/host_cpp/oracle_driver.c: every entry point of the C oracle.
+  * tests/host_cpp/oracle_driver.c: every entry point of the C oracle;
+  * tests/host_cpp/test_server_threads.cpp: the serving server's threads
+    (worker, submitting, polling, cancelling and waiting clients, a statistics
+    reader) with the HIP runtime and the kernel launchers replaced by
+    tests/host_cpp/fake_hip.cpp, plus an injected launch failure and
+    allocation failures at creation.
 The GPU parts of the library cannot run under these sanitizers (GPU
 AddressSanitizer is not available), so only host code is covered here.
 """
@@ -54,6 +61,32 @@ def test_oracle_sanitized(tmp_path):
     exe = str(tmp_path / "oracle_driver")
     srcs = [os.path.join(ROOT, "oracle", f) for f in ("pnx_ref.c", "pnx_ref_material.c", "pnx_ref_records.c")]
     _run(["gcc", "-std=c99", *SAN, "tests/host_cpp/oracle_driver.c", *srcs, "-lm", "-o", exe], exe)
+
+
+SERVER_SRCS = [os.path.join(CSRC, f) for f in ("pnx_server.cpp", "pnx_host.cpp", "pnx_setup.cpp", "pnx_coef.cpp")] + [
+    "tests/host_cpp/fake_hip.cpp", "tests/host_cpp/test_server_threads.cpp"]
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+
+def test_server_threads_sanitized(tmp_path):
+    exe = str(tmp_path / "server_asan")
+    _run(["g++", "-std=c++17", *SAN, *HIP_HOST, "-fconstexpr-ops-limit=1000000000", "-fconstexpr-loop-limit=10000000",
+          *SERVER_SRCS, "-lpthread", "-o", exe], exe)
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs the ROCm clang (its ThreadSanitizer runtime)")
+def test_server_threads_thread_sanitizer(tmp_path):
+    """clang's ThreadSanitizer (g++ 11's does not intercept the
+    pthread_cond_clockwait that std::condition_variable::wait_for calls, and
+    reports the waits as a double lock)."""
+    exe = str(tmp_path / "server_tsan")
+    subprocess.check_call([CLANG, "-std=c++17", "-O1", "-g", "-fsanitize=thread", "-fno-omit-frame-pointer", *HIP_HOST,
+                           "-fconstexpr-steps=100000000", *SERVER_SRCS, "-lpthread", "-o", exe], cwd=ROOT)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1")
+    for _ in range(3):  # the threads interleave differently on every run
+        out = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=300)
+        print(out.stdout[-2000:], out.stderr[-6000:])
+        assert out.returncode == 0 and out.stdout.strip().endswith("ok")
 
 
 def test_request_validation_refuses_malformed_descriptors():
