@@ -481,14 +481,19 @@ template <bool WIDE>
 __device__ __forceinline__ int tile_thread() { return WIDE ? (int)threadIdx.x & (kBlock - 1) : (int)threadIdx.x; }
 
 // Cell (i, j, k) of this thread in tile `tile` (tile_x(DIM) x 256/tile_x(DIM)
-// cells of one z-plane); false for threads past the grid's edge.
+// cells of one z-plane); false for threads past the grid's edge.  Tiles are
+// numbered x-fastest, then z, then y: a row of tiles marches through the
+// planes, so the neighbouring planes a 3D stencil reads were read by the
+// tiles just before (with the planes outermost, 3D half steps fetched
+// 1.5-1.75x their algorithmic bytes).  In 1D and 2D (Mz = 1) this is the
+// plain x-then-y order.
 template <int DIM, bool WIDE = false>
 __device__ __forceinline__ bool tile_cell(const Layout& L, int tile, int& i, int& j, int& k) {
   constexpr int TX = tile_x(DIM), TY = kBlock / TX;
   const int tx = tile % L.tiles_x;
   const int r = tile / L.tiles_x;
-  const int ty = r % L.tiles_y;
-  k = r / L.tiles_y;
+  k = r % L.Mz;
+  const int ty = r / L.Mz;
   i = tx * TX + tile_thread<WIDE>() % TX;
   j = ty * TY + tile_thread<WIDE>() / TX;
   return i < L.Mx && j < L.My;
@@ -917,7 +922,8 @@ __global__ __launch_bounds__(kBlock, rev_cw_min_blocks(N)) void rev_kernel_cw(Re
 
 // 3D form of the fused reverse pass with one class per wavefront.  Sub-tiles
 // are the 64-cell sub-tiles of the 2D form in every z-plane (sub-tile
-// item & 3 of cw tile item >> 2, cw tiles x-fastest, then planes).  Block
+// item & 3 of cw tile item >> 2, cw tiles x-fastest, then planes, then rows of
+// cw tiles, for the reuse of neighbouring planes as in tile_cell).  Block
 // item2 takes half h = item2 & 1 of the classes (h = 0: eee, eeo, eoe, eoo;
 // h = 1: oee, oeo, ooe, ooo; each half two odd and two even classes) at the
 // two sub-tiles 2 (q & 1) and 2 (q & 1) + 1 of cw tile and plane q >> 1,
@@ -931,10 +937,8 @@ __device__ __forceinline__ void rev_body_cw3(const RevArgs& a, int beam, int ite
   const int half = item2 & 1, q = item2 >> 1;
   constexpr int SUB_ROWS = 64 / kCwTileX;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int cw_tiles_y = (L.My + kCwTileX - 1) / kCwTileX;
   const int s0 = 2 * (q & 1), r = q >> 1;
-  const int tile = r % (L.cw_tiles_x * cw_tiles_y), k = r / (L.cw_tiles_x * cw_tiles_y);
-  const int tx = tile % L.cw_tiles_x, ty = tile / L.cw_tiles_x;
+  const int tx = r % L.cw_tiles_x, k = (r / L.cw_tiles_x) % L.Mz, ty = r / L.cw_tiles_x / L.Mz;
   const int i = tx * kCwTileX + lane % kCwTileX;
   const int j0 = ty * kCwTileX + lane / kCwTileX;
   const int s_odd = s0 + (wave >> 1), s_even = s0 + 1 - (wave >> 1);
