@@ -1,10 +1,10 @@
 // March kernel: one workgroup advances one 1D request through a slice of its
 // forward+adjoint sequence (march_next), with the half steps, density
 // pullbacks, checkpoint copies and the intensity reduction separated by
-// workgroup barriers instead of kernel boundaries.  A 1D request has at most
-// a few hundred cells (one or two tiles of 256), so a workgroup holds all of
-// it and a step costs a few barrier round trips rather than seven launches.
-// A workgroup of 2 x 256 threads runs two tiles of a pass side by side, as the
+// workgroup barriers instead of kernel boundaries.  A 1D request has a few
+// hundred cells (a few tiles of 256), so a workgroup holds all of it and a
+// step costs a few barrier round trips rather than seven launches.  A
+// workgroup of 2 x 256 threads runs two tiles of a pass side by side, as the
 // grid kernels' blocks do, instead of one after the other.  The device
 // functions are the ones the grid kernels use, in the same order and with the
 // same tiles, so the results are bitwise those of the grid path for either
