@@ -706,8 +706,9 @@ def main():
                     help="share of the HBM budget the server may hold in taped requests (0: always checkpoint)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(how the path would shard over GPUs is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the served path runs on a single MI355X "
+              "(beams and requests are independent solves; over several GPUs each process would serve "
+              "its own share with no exchange during the march)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "serve":
         main_serve(args)
